@@ -1,0 +1,123 @@
+// fx_internal.h -- device-side data structures shared by the HIP sources of
+// libfluxos_hip.so.  Not part of the public ABI (include/fluxos_hip.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/fluxos_hip.h"
+
+// Step control block, one per domain, resident in device memory.  The CFL
+// kernel's last block writes it; every later kernel of the step reads it, so
+// a step needs no host round trip.
+struct fx_ctl {
+    double tim;          // ds.tim after the current step's CFL
+    double dtfl;         // ds.dtfl of the current step
+    double hpall;        // max wet depth seen by the CFL (solver gate)
+    double print_next;   // next print time
+    double t_end;        // stop once tim > t_end (main.cpp:271)
+    double meteoi;       // melt depth of this step [m] (before multiplier)
+    double inflowi;      // inflow depth of this step [m]
+    double cmelt[FX_MAX_CHEM];
+    double cin[FX_MAX_CHEM];
+    // FP64 water ledger (extension): melt volume added to non-weir cells and
+    // net volume entering non-weir cells through faces shared with weir cells
+    double led_melt, led_melt_c;      // Kahan sum and compensation
+    double led_weir, led_weir_c;
+    int32_t it;          // solver steps so far
+    int32_t gate;        // hpall != 0 for the current step
+    int32_t halt;        // steps are no-ops while set
+    int32_t pending_halt;
+    int32_t steps;       // iterations run since the host last cleared them
+    int32_t solver_steps;
+    int32_t status;      // bit2: non-finite value seen
+    uint32_t blocks_done;// last-block-done counter of the CFL reduction
+    uint32_t ade_ticket; // block tickets of the ADE wavefront
+    uint32_t ade_error;  // the wavefront's bounded wait expired
+    uint32_t pad[2];
+};
+
+// launch mode bits
+#define FXM_GATED 1u     // honour ctl->halt / ctl->gate (device-resident loop)
+#define FXM_MELT  2u     // apply add_meteo in the melt kernel
+#define FXM_LOOP1 4u     // apply hydro loop 1 in the melt kernel
+#define FXM_INFLOW 8u    // apply add_inflow in the melt kernel
+
+// FP64 parity state (reference dtypes; flags as bytes on the device)
+struct fx_state64 {
+    double *z, *zb, *h, *qx, *qy, *ks;
+    double *fe1, *fe2, *fe3, *fn1, *fn2, *fn3;
+    double *twet, *h0;
+    uint8_t *ldry, *ldry_prev, *weir;
+    double *conc[FX_MAX_CHEM];
+    double *conc_tmp;          // pass-1 copy used by the exact ADE
+};
+
+// FP32-mixed state: depth relative to the bed in FP64 (ping-pong), discharges
+// and fluxes in FP32, bed in FP32, per-cell flag byte.
+struct fx_state32 {
+    double *d[2];              // z - zb, signed (the reference keeps deficits)
+    float *qx[2], *qy[2];
+    float *zb;                 // bed elevation (FP32)
+    float *f6;                 // persistent fluxes fe1,fe2,fe3,fn1,fn2,fn3 (S1), SoA
+    float *ff[2];              // fe1/fn1 of the step for the fast ADE (SoA pair)
+    float *mass[FX_MAX_CHEM];  // solute mass per unit area (c*h) for ADE fast
+    float *mass_tmp[FX_MAX_CHEM];
+    uint8_t *flags;            // bit0 weir, bit1 stored fluxes nonzero, bit2 NODATA bed
+    double *zb64;              // bed in FP64 (output conversion, fronts)
+    float *twet;
+    int cur;                   // index of the current d/qx/qy buffer
+};
+
+#define FXF_WEIR   1u
+#define FXF_STALE  2u
+#define FXF_NODATA 4u
+
+struct fx_domain_s {
+    fx_domain_desc desc;
+    int32_t mrows, mcols;
+    int64_t ncell;             // mrows*mcols
+    hipStream_t stream;
+    fx_ctl *ctl;               // device
+    fx_ctl *ctl_host;          // pinned mirror
+    double *partials;          // CFL block partials (2 per block)
+    int32_t npartials;
+    uint32_t *ade_progress;    // per-block progress of the ADE wavefront
+    int32_t ade_blocks;
+    float *meteo; int32_t meteo_nrows, meteo_ncols;
+    float *melt_mult;
+    float *inflow; int32_t inflow_nrows, inflow_ncols;
+    int64_t inflow_irow, inflow_icol;
+    bool has_static;
+    bool rates_nonneg;
+    fx_state64 s64;
+    fx_state32 s32;
+    std::vector<double> zb_host;   // FP64 bed (FP32 mode output conversion)
+    hipGraphExec_t graph;      // one captured iteration of the loop
+    bool graph_ok;
+    int64_t bytes;
+};
+
+// error helpers (fx_capi.hip)
+int fx_fail(int code, const std::string &msg);
+int fx_hip_check(hipError_t e, const char *what);
+#define FX_HIP(call) do { hipError_t e_ = (call); if (e_ != hipSuccess) return fx_hip_check(e_, #call); } while (0)
+
+// launchers implemented in the kernel sources
+int fx64_launch_cfl(fx_domain d, unsigned mode);
+int fx64_launch_melt(fx_domain d, unsigned mode);
+int fx64_launch_flux(fx_domain d, unsigned mode);
+int fx64_launch_update(fx_domain d, unsigned mode);
+int fx64_launch_ade_exact(fx_domain d, unsigned mode, int ichem, int it_override);
+int fx_launch_forcing(fx_domain d);
+int fx_launch_step_end(fx_domain d);
+int fx64_launch_volume(fx_domain d, double *vol_out);
+
+int fx32_init_state(fx_domain d);
+int fx32_launch_step(fx_domain d);
+int fx32_launch_cfl_initial(fx_domain d);
+int fx32_get_field(fx_domain d, int field, void *host);
+int fx32_set_field(fx_domain d, int field, const void *host);
+int fx32_launch_volume(fx_domain d, double *vol_out);

@@ -1,0 +1,175 @@
+/*
+ * fluxos_hip.h -- C ABI of libfluxos_hip.so, the MI355X (gfx950) time step of
+ * FLUXOS-OVERLAND.
+ *
+ * The reference has no plugin or FFI layer: its hot path is a set of C++ free
+ * functions taking `GlobVar&`.  Each entry point below names the reference
+ * seam it replaces.  Host buffers use the reference's own memory layout
+ * (arma::Mat, column-major, (NROWS+2) x (NCOLS+2), index irow + icol*MROWS,
+ * GlobVar.h:31-73), so a driver can hand over Mat::memptr() directly: double
+ * for state grids, float for the flag grids (ldry, innerNeumannBCWeir,
+ * ldry_prev).  Buffers are copied at the call and never retained.
+ *
+ * Errors: every call returns FX_OK (0) or a negative FX_ERR_* code and sets a
+ * thread-local message readable with fx_last_error().  Nothing calls exit().
+ *
+ * Threading: one host thread per domain at a time.  Each domain owns one HIP
+ * stream on its device; several domains may share a device.
+ */
+#ifndef FLUXOS_HIP_H
+#define FLUXOS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FX_MAX_CHEM 8
+
+/* status codes */
+#define FX_OK               0
+#define FX_ERR_ARG         -1   /* invalid argument                         */
+#define FX_ERR_HIP         -2   /* HIP runtime error                        */
+#define FX_ERR_STATE       -3   /* call not valid in the domain's state     */
+#define FX_ERR_UNSUPPORTED -4   /* input the reference leaves undefined     */
+#define FX_ERR_NONFINITE   -5   /* NaN/Inf found in the state               */
+#define FX_ERR_IO          -6   /* file could not be read or written        */
+#define FX_ERR_NOMEM       -7   /* device memory exhausted                  */
+#define FX_ERR_NOGPU       -8   /* no HIP device available                  */
+
+/* precision of the device state */
+#define FX_FP64_PARITY 0    /* the reference's FP64 arithmetic, bit for bit   */
+#define FX_FP32_MIXED  1    /* FP32 fluxes, FP64 depth/mass/dt                */
+
+/* advection-dispersion mode */
+#define FX_ADE_OFF   0
+#define FX_ADE_EXACT 1      /* the reference's sequential sweep (wavefront)  */
+#define FX_ADE_FAST  2      /* fully parallel (Jacobi) face-flux form        */
+
+typedef struct fx_domain_s *fx_domain;
+
+/* Replaces the scalars of GlobVar (GlobVar.h:75-104) plus main.cpp:132-135. */
+typedef struct fx_domain_desc {
+    int32_t nrows, ncols;        /* NROWS, NCOLS of the DEM                   */
+    int32_t dxy;                 /* CELLSIZE, an integer (read_functions.cpp:137) */
+    int32_t nchem;               /* solutes (1 without OpenWQ, main.cpp:244-249) */
+    double hdry;                 /* ROUGNESS_HEIGHT = ks(1,1) (main.cpp:259)  */
+    double nodata;               /* NODATA_VALUE of the DEM header            */
+    float gacc, cfl, cvdef, nuem;/* 9.80665, 1, 0.07, 1.793e-6 in the reference */
+    double d_coef[FX_MAX_CHEM];  /* D_COEF per solute                         */
+    int32_t ade_solver;          /* ADE_TRANSPORT.STATUS                      */
+    int32_t precision;           /* FX_FP64_PARITY | FX_FP32_MIXED            */
+    int32_t ade_mode;            /* FX_ADE_*                                  */
+    int32_t device;              /* HIP device ordinal                        */
+} fx_domain_desc;
+
+/* state fields for fx_set_field / fx_get_field (GlobVar.h:81-92) */
+#define FX_F_Z          0   /* double  water surface elevation z          */
+#define FX_F_H          1   /* double  depth h                             */
+#define FX_F_QX         2   /* double  qx                                  */
+#define FX_F_QY         3   /* double  qy                                  */
+#define FX_F_US         4   /* double  us (always zero in the reference)   */
+#define FX_F_LDRY       5   /* float   ldry                                */
+#define FX_F_LDRY_PREV  6   /* float   ldry_prev                           */
+#define FX_F_FE1        7   /* double  fe_1 .. fe_3, fn_1 .. fn_3          */
+#define FX_F_FE2        8
+#define FX_F_FE3        9
+#define FX_F_FN1       10
+#define FX_F_FN2       11
+#define FX_F_FN3       12
+#define FX_F_TWET      13   /* double  twetimetracer                      */
+#define FX_F_H0        14   /* double  h0                                  */
+#define FX_F_ZB        15   /* double  zb (read only; set by fx_set_static)*/
+#define FX_F_CONC0     32   /* double  conc_SW[k] is FX_F_CONC0 + k        */
+
+/* Per-call step summary of fx_advance. */
+typedef struct fx_step_info {
+    double tim;          /* ds.tim after the last step                      */
+    double dtfl;         /* ds.dtfl of the last step                        */
+    double hpall;        /* hpall of the last step (solver gate)            */
+    double print_next;   /* print time the call was given                   */
+    int32_t it;          /* solver-step counter (main.cpp:387)              */
+    int32_t steps;       /* loop iterations executed by this call           */
+    int32_t solver_steps;/* of which ran the solver (hpall != 0)            */
+    int32_t status;      /* bit0 print due, bit1 tim > t_end, bit2 NaN/Inf  */
+} fx_step_info;
+
+/* ---- lifetime ----------------------------------------------------------
+ * Replaces `GlobVar ds(NROWS+2, NCOLS+2)` (main.cpp:125-127, GlobVar.h:31-74):
+ * allocates the domain's device state.                                    */
+int fx_domain_create(const fx_domain_desc *desc, fx_domain *out);
+int fx_domain_destroy(fx_domain dom);
+/* Device bytes the domain holds (for admission by memory budget). */
+int64_t fx_domain_bytes(const fx_domain_desc *desc);
+
+/* ---- state transfer -----------------------------------------------------
+ * Static grids after read_geo + initiation (read_functions.cpp:221-223,
+ * initiate.cpp:43-90,192-228): zb and ks double, weir = innerNeumannBCWeir
+ * float.  Every cell of the outer interior ring must be a weir cell: the
+ * reference's weir branch reads an uninitialised `volrat` otherwise
+ * (solver_drydomain.cpp:36,178-185; solver_wetdomain.cpp:37,300-307). */
+int fx_set_static(fx_domain dom, const double *zb, const double *ks, const float *weir);
+int fx_set_field(fx_domain dom, int32_t field, const void *host);
+int fx_get_field(fx_domain dom, int32_t field, void *host);
+/* ds.tim, ds.dtfl and the solver-step counter `it` (main.cpp:72,271-325). */
+int fx_set_clock(fx_domain dom, double tim, double dtfl, int32_t it);
+int fx_get_clock(fx_domain dom, double *tim, double *dtfl, int32_t *it);
+
+/* ---- forcing ---------------------------------------------------------------
+ * Meteo table as read by read_meteo (read_functions.cpp:237-282): nrows rows
+ * of ncols = 2+nchem floats, row-major (t [s], rate [mm/day], conc_k), row 0
+ * being the empty header slot.  melt_mult (optional, padded grid of floats)
+ * scales the melt per cell; NULL = uniform as in the reference.           */
+int fx_set_meteo(fx_domain dom, int32_t nrows, int32_t ncols, const float *table,
+                 const float *melt_mult);
+/* Inflow table as read by read_inflow (read_functions.cpp:285-326): rows
+ * from 0, and the discharge cell (read_functions.cpp:228-231).            */
+int fx_set_inflow(fx_domain dom, int32_t nrows, int32_t ncols, const float *table,
+                  int64_t irow, int64_t icol);
+
+/* ---- granular seams (FX_FP64_PARITY domains) --------------------------------
+ * fx_courant_condition: the inline CFL block main.cpp:300-323 plus
+ * `ds.tim += ds.dtfl` (main.cpp:325).  Returns dtfl and hpall.            */
+int fx_courant_condition(fx_domain dom, double print_next, double *dtfl, double *hpall);
+/* add_meteo, common.cpp:164-273 (common.h:50). */
+int fx_add_meteo(fx_domain dom);
+/* add_inflow, common.cpp:277-390 (common.h:69). */
+int fx_add_inflow(fx_domain dom);
+/* hydrodynamics_calc, hydrodynamics_calc.cpp:27-157 (hydrodynamics_calc.h:24),
+ * including solver_dry / solver_wet (solver_drydomain.h:25, solver_wetdomain.h:25). */
+int fx_hydrodynamics_calc(fx_domain dom);
+/* adesolver_calc, ADEsolver_calc.cpp:27-291 (ADEsolver_calc.h:25). */
+int fx_adesolver_calc(fx_domain dom, int32_t it, int32_t ichem);
+
+/* ---- device-resident loop ----------------------------------------------------
+ * Iterations of main.cpp:271-406 without host round trips: each iteration is
+ * CFL -> tim += dt -> forcing -> (if hpall != 0) hydrodynamics + ADE.  Runs
+ * while tim <= t_end and stops after the iteration that reaches print_next
+ * (main.cpp:444) or after max_steps iterations.                           */
+int fx_advance(fx_domain dom, double t_end, double print_next, int32_t max_steps,
+               fx_step_info *info);
+/* Water volume of the non-weir cells and the running ledger of melt input and
+ * weir-face exchange, all accumulated in FP64 (extension; no counterpart in
+ * the reference).  out[0]=volume, out[1]=melt in, out[2]=net weir inflow. */
+int fx_mass_balance(fx_domain dom, double out[3]);
+int fx_synchronize(fx_domain dom);
+
+/* ---- host input layer (no GPU needed) -----------------------------------------
+ * read_geo on a parsed DEM (read_functions.cpp:145-225): dem has NROWS lines
+ * of NCOLS values in file order.  Outputs are padded grids.               */
+int fx_read_geo(int32_t nrows, int32_t ncols, double nodata, const float *dem,
+                double ks_input, double *zb, double *ks, float *weir);
+/* initiation, cold start (initiate.cpp:27-231): zb in place (vertex to cell
+ * centre, ghost extrapolation) and the initial z, h, ldry, conc grids.    */
+int fx_initiation(int32_t nrows, int32_t ncols, int32_t ade_solver, double *zb,
+                  double *z, double *h, float *ldry, double *conc0);
+
+const char *fx_last_error(void);
+const char *fx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
