@@ -1,0 +1,243 @@
+#!/usr/bin/env python3
+"""Benchmark of the FLUXOS-OVERLAND time step on one MI355X.
+
+Workload (the single large domain the metric is quoted on): a 4096 x 4096
+basin, dx = 2 m, diamond-square fractal DEM (roughness 0.6, relief 8 m) over a
+0.3 % slope, melt ramping 0 -> 60 mm/day over 6 h, two solutes.  The run is
+spun up on the device past the time the melt wets the basin; then W warm-up
+steps and K timed steps of the hot path (CFL -> forcing -> fluxes -> update ->
+ADE per solute) run device-resident.  A step is one loop iteration of the
+reference's main.cpp:271-406 with the solver active; a cell-update is one
+interior cell advanced by one such step.
+
+Prints ONE JSON line (rank 0): metric, value (cell-updates/s), roofline of the
+dominant kernel (HIP events on the library's own stream), and the reference
+CPU path (the oracle, OpenMP) timed on this host on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "cell-updates/s FP32-mixed vs FP64 at <=1e-3 m h error; p50/p99 scenario latency"
+
+# algorithmic bytes per cell-update (read d|z, qx, qy, zb; write d|z, qx, qy;
+# plus c read+write per solute) -- flags, masks, persistent fluxes and
+# wet-time tracer are implementation overhead and not counted
+def b_alg(precision: str, nchem: int) -> int:
+    return (28 + 8 * nchem) if precision == "fp32" else (56 + 16 * nchem)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def spin_up(dom, t_spin: float, max_steps: int) -> tuple[float, int]:
+    """Advance the device loop until simulated time t_spin (bounded)."""
+    done = 0
+    t = 0.0
+    print_next = 600.0
+    while t < t_spin and done < max_steps:
+        info = dom.advance(1e30, print_next, max_steps - done)
+        done += info.steps
+        t = info.tim
+        if t >= print_next:
+            print_next += 600.0
+        if info.steps == 0:
+            break
+    return t, done
+
+
+def cpu_baseline(dom, sc, nsteps: int, threads: int | None):
+    """The oracle (the reference's FP64 OpenMP path restated in C) on a
+    bounded sample: nsteps loop iterations of the same domain from the
+    device's current state."""
+    from oracle.oracle import OracleDomain
+    from tests.helpers import STATE_D, STATE_F
+    if threads:
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+    o = OracleDomain(sc.nrows, sc.ncols, sc.dxy, sc.hdry, sc.nchem, sc.d_coef, True)
+    from fluxos_cpp_amd import capi
+    st = capi.prepare(sc)
+    o.g["zb"][...] = st.zb
+    o.g["ks"][...] = st.ks
+    o.g["weir"][...] = st.weir
+    for k in STATE_D:
+        o.g[k][...] = dom.get(k)
+    for k in STATE_F:
+        o.g[k][...] = dom.get(k)
+    for k in range(sc.nchem):
+        o.conc[k][...] = dom.get_conc(k)
+    t, dt, it = dom.clock()
+    o.tim, o.dtfl = t, dt
+    o.it.value = it
+    o.set_meteo(sc.meteo)
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        o.step(1e30)
+    el = time.perf_counter() - t0
+    return sc.nrows * sc.ncols * nsteps / el, el, o.lib.orc_num_threads()
+
+
+def snapshot(src, fields_only=False) -> dict:
+    """Every grid a domain needs to restart, in the reference layout."""
+    from fluxos_cpp_amd import capi
+    out = {k: src.get(k) for k in ("z", "h", "qx", "qy", "fe1", "fe2", "fe3", "fn1", "fn2", "fn3",
+                                   "twet", "ldry", "ldry_prev", "h0")}
+    out["conc"] = [src.get_conc(k) for k in range(src.nchem)]
+    out["clock"] = src.clock()
+    return out
+
+
+def restore(dst, snap: dict) -> None:
+    for k, v in snap.items():
+        if k in ("conc", "clock"):
+            continue
+        dst.set(k, v)
+    h = snap.get("h")
+    for k, c in enumerate(snap["conc"]):
+        dst.set_conc(k, c)
+    t, dt, it = snap["clock"]
+    dst.set_clock(t, dt, it)
+
+
+def timed(dom, steps: int):
+    dom.synchronize()
+    dom.profile(True)
+    dom.synchronize()
+    t0 = time.perf_counter()
+    info = dom.advance(1e30, 1e30, steps)
+    dom.synchronize()
+    el = time.perf_counter() - t0
+    from fluxos_cpp_amd import capi
+    kt = {}
+    for kid, name in capi.KERNEL_NAMES.items():
+        ms, n = dom.profile_read(kid)
+        if n:
+            kt[name] = (ms, n)
+    dom.profile(False)
+    return info, el, kt
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--fp64-steps", type=int, default=10)
+    ap.add_argument("--spinup-t", type=float, default=3600.0)
+    ap.add_argument("--spinup-max-steps", type=int, default=20000)
+    ap.add_argument("--accuracy-span", type=float, default=120.0)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the path is a "
+              "single-device kernel; multi-GPU sharding is described in DESIGN.md", file=sys.stderr)
+        return 2
+
+    from fluxos_cpp_amd import capi, gen
+    capi.load()
+    t0 = time.perf_counter()
+    sc = gen.scenario_c2(args.n)
+    init = capi.prepare(sc)
+    log(f"[bench] generated {args.n}^2 basin in {time.perf_counter() - t0:.1f}s")
+    cells = sc.nrows * sc.ncols
+
+    # FP32-mixed: spin up past the wet-up, then warm-up and timed steps
+    d32 = capi.HipDomain.from_scenario(sc, precision=capi.FP32_MIXED, ade=True, init=init)
+    t0 = time.perf_counter()
+    t_sim, nsp = spin_up(d32, args.spinup_t, args.spinup_max_steps)
+    log(f"[bench] fp32 spin-up to t={t_sim:.1f}s in {nsp} steps, {time.perf_counter() - t0:.1f}s wall")
+    snap = snapshot(d32)
+    d32.advance(1e30, 1e30, args.warmup)
+    info, el, kt = timed(d32, args.steps)
+    if info.solver_steps != args.steps:
+        log(f"[bench] warning: {info.steps} steps, {info.solver_steps} with the solver")
+    value = cells * info.solver_steps / el
+    ms_step = el * 1e3 / max(1, info.steps)
+    log("[bench] fp32 kernels: " + ", ".join(f"{k}={v[0] / max(1, v[1]) * 1e3:.1f}us x{v[1]}" for k, v in kt.items()))
+
+    # roofline of the dominant kernel: the fused step (B_alg = 28 B per cell-update)
+    dom_name = max(kt, key=lambda k: kt[k][0])
+    avg_ms = kt["k32_step"][0] / kt["k32_step"][1]
+    achieved = cells * b_alg("fp32", 0) / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k32_step",
+                "bytes_per_cell": b_alg("fp32", 0), "avg_us": round(avg_ms * 1e3, 2),
+                "dominant_kernel": dom_name}
+    step_ms = sum(v[0] for v in kt.values()) / max(1, info.steps)
+    step_frac = cells * b_alg("fp32", sc.nchem) / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+
+    # FP64 parity path from the same snapshot, and the depth error at equal time
+    fp64 = None
+    err = None
+    d64 = None
+    if not args.no_fp64:
+        d64 = capi.HipDomain.from_scenario(sc, precision=capi.FP64_PARITY, ade=True, init=init)
+        restore(d64, snap)
+        info64, el64, kt64 = timed(d64, args.fp64_steps)
+        log("[bench] fp64 kernels: " + ", ".join(f"{k}={v[0] / max(1, v[1]) * 1e3:.1f}us x{v[1]}" for k, v in kt64.items()))
+        fp64 = {"value": round(cells * info64.solver_steps / el64, 1),
+                "ms_per_step": round(el64 * 1e3 / max(1, info64.steps), 3),
+                "steps": info64.steps}
+        # equal-time accuracy: both paths from the snapshot to snapshot + span
+        restore(d64, snap)
+        d32b = capi.HipDomain.from_scenario(sc, precision=capi.FP32_MIXED, ade=True, init=init)
+        restore(d32b, snap)
+        t_to = snap["clock"][0] + args.accuracy_span
+        i64 = d64.advance(1e30, t_to, 10 ** 7)
+        i32 = d32b.advance(1e30, t_to, 10 ** 7)
+        h64, h32 = d64.get("h")[1:-1, 1:-1], d32b.get("h")[1:-1, 1:-1]
+        w64 = int((d64.get("ldry")[1:-1, 1:-1] == 0).sum())
+        w32 = int((d32b.get("ldry")[1:-1, 1:-1] == 0).sum())
+        err = {"max_abs_h_m": float(np.nanmax(np.abs(h64 - h32))), "t_from": round(snap["clock"][0], 2),
+               "t_to": round(t_to, 2), "steps_fp64": i64.steps, "steps_fp32": i32.steps,
+               "wet_fp64": w64, "wet_fp32": w32}
+        v, melt, weir = d32.mass_balance()
+        d32b.close()
+
+    cpu = None
+    if not args.no_cpu:
+        src = d64 if d64 is not None else d32
+        if d64 is not None:
+            restore(d64, snap)
+        rate, cel, thr = cpu_baseline(src, sc, args.cpu_steps, args.cpu_threads or None)
+        cpu = {"value": round(rate, 1), "unit": "cell-updates/s", "cores": thr, "kind": "port",
+               "sample": f"{args.cpu_steps} loop iterations of the {args.n}x{args.n} basin from the "
+                         f"GPU run's state at t={snap['clock'][0]:.0f}s ({cel:.1f}s wall; hydro loops "
+                         f"OpenMP, CFL/melt/ADE serial as in the reference)"}
+
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (f64 depth, dt, mass)", "data": "synthetic",
+        "config": {"workload": f"single domain {args.n}x{args.n}, dx=2 m, fractal DEM, melt 0-60 mm/day, "
+                               f"2 solutes, FP32-mixed with fast ADE", "cells": cells, "nchem": sc.nchem,
+                   "spinup_t": round(t_sim, 1)},
+        "roofline": roofline,
+        "step_roofline_frac": round(step_frac, 4),
+        "fp64_parity": fp64, "fp32_vs_fp64": err,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -38,7 +38,12 @@ EXPORTS = ("fx_domain_create", "fx_domain_destroy", "fx_domain_bytes", "fx_set_s
            "fx_set_field", "fx_get_field", "fx_set_clock", "fx_get_clock", "fx_set_meteo",
            "fx_set_inflow", "fx_courant_condition", "fx_add_meteo", "fx_add_inflow",
            "fx_hydrodynamics_calc", "fx_adesolver_calc", "fx_advance", "fx_mass_balance",
-           "fx_synchronize", "fx_read_geo", "fx_initiation", "fx_last_error", "fx_version")
+           "fx_synchronize", "fx_read_geo", "fx_initiation", "fx_last_error", "fx_version",
+           "fx_profile", "fx_profile_read")
+K_CFL, K_MELT, K_FLUX, K_UPDATE, K_ADE_PASS1, K_ADE_SWEEP, K_STEP32, K_ADE32, K_CFL32 = range(9)
+KERNEL_NAMES = {K_CFL: "k64_cfl", K_MELT: "k64_melt", K_FLUX: "k64_flux", K_UPDATE: "k64_update",
+                K_ADE_PASS1: "k_ade_pass1", K_ADE_SWEEP: "k_ade_sweep", K_STEP32: "k32_step",
+                K_ADE32: "k32_ade", K_CFL32: "k32_cfl"}
 
 
 class FxError(RuntimeError):
@@ -98,6 +103,8 @@ def load(path: Path | None = None) -> C.CDLL:
         "fx_synchronize": ([vp], C.c_int),
         "fx_read_geo": ([C.c_int32, C.c_int32, C.c_double, _F, C.c_double, _D, _D, _F], C.c_int),
         "fx_initiation": ([C.c_int32, C.c_int32, C.c_int32, _D, _D, _D, _F, _D], C.c_int),
+        "fx_profile": ([vp, C.c_int32], C.c_int),
+        "fx_profile_read": ([vp, C.c_int32, _D, C.POINTER(C.c_int64)], C.c_int),
         "fx_last_error": ([], C.c_char_p),
         "fx_version": ([], C.c_char_p),
     }
@@ -295,6 +302,14 @@ class HipDomain:
         out = (C.c_double * 3)()
         _check(self.lib.fx_mass_balance(self.h, out))
         return out[0], out[1], out[2]
+
+    def profile(self, on: bool) -> None:
+        _check(self.lib.fx_profile(self.h, 1 if on else 0))
+
+    def profile_read(self, kernel: int):
+        ms, n = C.c_double(), C.c_int64()
+        _check(self.lib.fx_profile_read(self.h, int(kernel), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
 
     def synchronize(self) -> None:
         _check(self.lib.fx_synchronize(self.h))

@@ -260,11 +260,15 @@ int fx64_launch_ade_exact(fx_domain d, unsigned mode, int ichem, int it_override
     const int64_t mm = (int64_t)d->mrows * d->mcols;
     const int nblocks = (P.ncols + ADE_W - 1) / ADE_W;
     if (nblocks > d->ade_blocks) return fx_fail(FX_ERR_STATE, "ADE block buffers too small");
+    fx_prof_begin(d, FX_K_ADE_PASS1);
     hipLaunchKernelGGL(k_ade_pass1, dim3((unsigned)((mm + 255) / 256)), dim3(256), 0, d->stream,
                        P, d->s64, ichem, d->ctl, mode, d->ade_progress, nblocks);
+    fx_prof_end(d, FX_K_ADE_PASS1);
     FX_HIP(hipGetLastError());
     double *qedge = (double *)(d->ade_progress + ((d->ade_blocks + 63) / 64) * 64);
+    fx_prof_begin(d, FX_K_ADE_SWEEP);
     hipLaunchKernelGGL(k_ade_sweep, dim3(nblocks), dim3(ADE_W), 0, d->stream,
                        P, d->s64, ichem, d->ctl, mode, d->ade_progress, qedge);
+    fx_prof_end(d, FX_K_ADE_SWEEP);
     return fx_hip_check(hipGetLastError(), "k_ade_sweep");
 }

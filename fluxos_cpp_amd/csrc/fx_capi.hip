@@ -28,6 +28,51 @@ int fx_hip_check(hipError_t e, const char *what)
     return fx_fail(FX_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+void fx_prof_begin(fx_domain d, int id)
+{
+    if (!d->prof.on) return;
+    fx_prof &p = d->prof;
+    if (p.pool.size() < 2) {
+        for (int i = 0; i < 64; i++) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return;
+            p.pool.push_back(e);
+        }
+    }
+    fx_prof::rec r;
+    r.id = id;
+    r.a = p.pool.back(); p.pool.pop_back();
+    r.b = p.pool.back(); p.pool.pop_back();
+    (void)hipEventRecord(r.a, d->stream);
+    p.pending.push_back(r);
+}
+
+void fx_prof_end(fx_domain d, int id)
+{
+    (void)id;
+    if (!d->prof.on || d->prof.pending.empty()) return;
+    (void)hipEventRecord(d->prof.pending.back().b, d->stream);
+    if (d->prof.pending.size() > 8192) (void)fx_prof_drain(d);
+}
+
+int fx_prof_drain(fx_domain d)
+{
+    fx_prof &p = d->prof;
+    if (p.pending.empty()) return FX_OK;
+    FX_HIP(hipStreamSynchronize(d->stream));
+    for (auto &r : p.pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess && r.id >= 0 && r.id < FX_K_COUNT) {
+            p.total_ms[r.id] += ms;
+            p.count[r.id] += 1;
+        }
+        p.pool.push_back(r.a);
+        p.pool.push_back(r.b);
+    }
+    p.pending.clear();
+    return FX_OK;
+}
+
 fx_params fx_make_params(fx_domain d)
 {
     fx_params P;
@@ -51,6 +96,7 @@ fx_params fx_make_params(fx_domain d)
     P.arbase = (double)((int64_t)D.dxy * (int64_t)D.dxy);
     P.dxy_u = (uint32_t)D.dxy;
     P.hdry = D.hdry;
+    P.ks0 = d->ks0;
     P.nodata = D.nodata;
     P.g = (double)D.gacc;
     P.cfl_dxy = D.cfl * (float)D.dxy;
@@ -81,9 +127,9 @@ int64_t fp64_bytes(const fx_domain_desc &D)
 int64_t fp32_bytes(const fx_domain_desc &D)
 {
     const int64_t mm = (int64_t)(D.nrows + 2) * (D.ncols + 2);
-    // d x2 (f64), qx,qy x2, zb, f6, ff x2, twet (f32), flags, zb64
-    int64_t b = mm * 16 + mm * 4 * 4 + mm * 4 + mm * 4 * 6 + mm * 4 * 2 + mm * 4 + mm + mm * 8;
-    if (D.ade_solver) b += mm * 4 * 2 * std::max(1, D.nchem);
+    // d x2 (f64), qx,qy x2, zb, f6 (6 planes), ff (2 planes), tw x2, flags
+    int64_t b = mm * 16 + mm * 4 * 4 + mm * 6 + mm * 4 * 6 + mm * 4 * 2 + mm * 4 * 2 + mm;
+    if (D.ade_solver) b += mm * 8 * std::max(1, D.nchem);
     return b;
 }
 
@@ -285,10 +331,12 @@ int fx_domain_destroy(fx_domain d)
     for (void *p : ptrs) dfree(p);
     for (int k = 0; k < FX_MAX_CHEM; k++) dfree(s.conc[k]);
     fx_state32 &t = d->s32;
-    void *ptrs32[] = {t.d[0], t.d[1], t.qx[0], t.qx[1], t.qy[0], t.qy[1], t.zb, t.f6, t.ff[0],
-                      t.ff[1], t.flags, t.zb64, t.twet};
+    void *ptrs32[] = {t.d[0], t.d[1], t.qx[0], t.qx[1], t.qy[0], t.qy[1], t.zb, t.zb_lo, t.f6, t.ff,
+                      t.flags, t.tw_acc, t.tw_t0, t.partials};
     for (void *p : ptrs32) dfree(p);
-    for (int k = 0; k < FX_MAX_CHEM; k++) { dfree(t.mass[k]); dfree(t.mass_tmp[k]); }
+    for (int k = 0; k < FX_MAX_CHEM; k++) { dfree(t.mass[0][k]); dfree(t.mass[1][k]); }
+    for (auto &r : d->prof.pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+    for (auto e : d->prof.pool) (void)hipEventDestroy(e);
     dfree(d->ctl);
     dfree(d->partials);
     dfree(d->ade_progress);
@@ -323,6 +371,7 @@ int fx_set_static(fx_domain d, const double *zb, const double *ks, const float *
     } else {
         // ks enters the FP32 path as the interior roughness: it must be uniform
         const double ks0 = ks[1 + (size_t)1 * d->mrows];
+        d->ks0 = ks0;
         for (int icol = 1; icol <= d->desc.ncols; icol++)
             for (int irow = 1; irow <= d->desc.nrows; irow++)
                 if (ks[irow + (size_t)icol * d->mrows] != ks0)
@@ -332,6 +381,7 @@ int fx_set_static(fx_domain d, const double *zb, const double *ks, const float *
         if ((rc = fx32_set_field(d, -1 /* weir */, w.data()))) return rc;
     }
     d->has_static = true;
+    d->melt_area_ok = false;
     return FX_OK;
 }
 
@@ -445,6 +495,7 @@ int fx_set_meteo(fx_domain d, int32_t nrows, int32_t ncols, const float *table, 
         FX_HIP(hipMalloc((void **)&d->melt_mult, d->ncell * sizeof(float)));
         FX_HIP(hipMemcpy(d->melt_mult, melt_mult, d->ncell * sizeof(float), hipMemcpyHostToDevice));
     }
+    d->melt_area_ok = false;
     if (d->graph_ok) { (void)hipGraphExecDestroy(d->graph); d->graph_ok = false; }
     return FX_OK;
 }
@@ -617,9 +668,32 @@ int fx_mass_balance(fx_domain d, double out[3])
     else rc = fx32_launch_volume(d, &vol);
     if (rc) return rc;
     if ((rc = pull_ctl(d))) return rc;
+    double pend = 0.0;
+    if (d->desc.precision == FX_FP32_MIXED && (rc = fx32_pending_weir(d, &pend))) return rc;
     out[0] = vol;
     out[1] = d->ctl_host->led_melt;
-    out[2] = d->ctl_host->led_weir;
+    out[2] = d->ctl_host->led_weir + pend;
+    return FX_OK;
+}
+
+int fx_profile(fx_domain d, int32_t on)
+{
+    int rc = enter(d);
+    if (rc) return rc;
+    if ((rc = fx_prof_drain(d))) return rc;
+    for (int i = 0; i < FX_K_COUNT; i++) { d->prof.total_ms[i] = 0.0; d->prof.count[i] = 0; }
+    d->prof.on = on != 0;
+    return FX_OK;
+}
+
+int fx_profile_read(fx_domain d, int32_t kernel, double *total_ms, int64_t *launches)
+{
+    int rc = enter(d);
+    if (rc) return rc;
+    if (kernel < 0 || kernel >= FX_K_COUNT) return fx_fail(FX_ERR_ARG, "kernel id out of range");
+    if ((rc = fx_prof_drain(d))) return rc;
+    if (total_ms) *total_ms = d->prof.total_ms[kernel];
+    if (launches) *launches = d->prof.count[kernel];
     return FX_OK;
 }
 

@@ -518,29 +518,37 @@ inline dim3 grid_of(const fx_params &P)
 int fx64_launch_cfl(fx_domain d, unsigned mode)
 {
     fx_params P = fx_make_params(d);
+    fx_prof_begin(d, FX_K_CFL);
     hipLaunchKernelGGL(k64_cfl, grid_of(P), dim3(BX, BY), 0, d->stream, P, d->s64, d->ctl,
                        d->partials, mode);
+    fx_prof_end(d, FX_K_CFL);
     return fx_hip_check(hipGetLastError(), "k64_cfl");
 }
 
 int fx64_launch_melt(fx_domain d, unsigned mode)
 {
     fx_params P = fx_make_params(d);
+    fx_prof_begin(d, FX_K_MELT);
     hipLaunchKernelGGL(k64_melt, grid_of(P), dim3(BX, BY), 0, d->stream, P, d->s64, d->ctl, mode);
+    fx_prof_end(d, FX_K_MELT);
     return fx_hip_check(hipGetLastError(), "k64_melt");
 }
 
 int fx64_launch_flux(fx_domain d, unsigned mode)
 {
     fx_params P = fx_make_params(d);
+    fx_prof_begin(d, FX_K_FLUX);
     hipLaunchKernelGGL(k64_flux, grid_of(P), dim3(BX, BY), 0, d->stream, P, d->s64, d->ctl, mode);
+    fx_prof_end(d, FX_K_FLUX);
     return fx_hip_check(hipGetLastError(), "k64_flux");
 }
 
 int fx64_launch_update(fx_domain d, unsigned mode)
 {
     fx_params P = fx_make_params(d);
+    fx_prof_begin(d, FX_K_UPDATE);
     hipLaunchKernelGGL(k64_update, grid_of(P), dim3(BX, BY), 0, d->stream, P, d->s64, d->ctl, mode);
+    fx_prof_end(d, FX_K_UPDATE);
     return fx_hip_check(hipGetLastError(), "k64_update");
 }
 

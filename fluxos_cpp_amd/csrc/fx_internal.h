@@ -36,7 +36,13 @@ struct fx_ctl {
     uint32_t blocks_done;// last-block-done counter of the CFL reduction
     uint32_t ade_ticket; // block tickets of the ADE wavefront
     uint32_t ade_error;  // the wavefront's bounded wait expired
-    uint32_t pad[2];
+    // FP32-mixed path: the step kernel leaves the next step's CFL inputs here
+    int32_t cur;         // current ping-pong buffer of d/qx/qy
+    int32_t nblk32;      // number of partials written by the last step kernel
+    double dt_raw;       // min over wet cells of the CFL bound (no print clip)
+    double s_wet;        // wet-time prefix sum [h] over solver steps
+    double s_wet_prev;
+    double melt_area;    // sum of the melt multiplier over non-weir bed cells
 };
 
 // launch mode bits
@@ -60,20 +66,32 @@ struct fx_state64 {
 struct fx_state32 {
     double *d[2];              // z - zb, signed (the reference keeps deficits)
     float *qx[2], *qy[2];
-    float *zb;                 // bed elevation (FP32)
-    float *f6;                 // persistent fluxes fe1,fe2,fe3,fn1,fn2,fn3 (S1), SoA
-    float *ff[2];              // fe1/fn1 of the step for the fast ADE (SoA pair)
-    float *mass[FX_MAX_CHEM];  // solute mass per unit area (c*h) for ADE fast
-    float *mass_tmp[FX_MAX_CHEM];
-    uint8_t *flags;            // bit0 weir, bit1 stored fluxes nonzero, bit2 NODATA bed
-    double *zb64;              // bed in FP64 (output conversion, fronts)
-    float *twet;
-    int cur;                   // index of the current d/qx/qy buffer
+    float *zb;                 // bed elevation, FP32 high part
+    _Float16 *zb_lo;           // zb - (double)zb_hi, so bed differences carry ~6e-8 m error
+    float *f6;                 // persistent fluxes fe1,fe2,fe3,fn1,fn2,fn3 (S1), SoA planes
+    float *ff;                 // fe1 / fn1 of the step for the fast ADE (two planes)
+    float *mass[2][FX_MAX_CHEM];  // solute mass per unit area (c*h), ping-pong
+    uint8_t *flags;            // FXF_* bits
+    float *tw_acc, *tw_t0;     // wet-time tracer: closed runs, start of the open run
+    double *partials;          // 3 per step-kernel block (dt, hpall, weir volume)
+    int64_t npart;
 };
 
-#define FXF_WEIR   1u
-#define FXF_STALE  2u
-#define FXF_NODATA 4u
+#define FXF_WEIR    1u   // innerNeumannBCWeir
+#define FXF_STALE   2u   // f6 holds nonzero fluxes for this cell
+#define FXF_NODATA  4u   // bed equals NODATA: no melt
+#define FXF_WETPREV 8u   // wet in loop 1 of the last solver step (open wet-time run)
+#define FXF_INNER  16u   // interior cell (1..NROWS x 1..NCOLS)
+
+// per-kernel HIP-event timing on the domain's stream (fx_profile)
+struct fx_prof {
+    bool on = false;
+    struct rec { int id; hipEvent_t a, b; };
+    std::vector<rec> pending;
+    std::vector<hipEvent_t> pool;
+    double total_ms[FX_K_COUNT] = {0};
+    int64_t count[FX_K_COUNT] = {0};
+};
 
 struct fx_domain_s {
     fx_domain_desc desc;
@@ -91,6 +109,8 @@ struct fx_domain_s {
     float *inflow; int32_t inflow_nrows, inflow_ncols;
     int64_t inflow_irow, inflow_icol;
     bool has_static;
+    double ks0;                // interior roughness (uniform in FP32-mixed domains)
+    bool melt_area_ok;         // ctl->melt_area is current
     bool rates_nonneg;
     fx_state64 s64;
     fx_state32 s32;
@@ -98,7 +118,13 @@ struct fx_domain_s {
     hipGraphExec_t graph;      // one captured iteration of the loop
     bool graph_ok;
     int64_t bytes;
+    fx_prof prof;
 };
+
+// bracket a launch with timing events when profiling is on
+void fx_prof_begin(fx_domain d, int id);
+void fx_prof_end(fx_domain d, int id);
+int fx_prof_drain(fx_domain d);
 
 // error helpers (fx_capi.hip)
 int fx_fail(int code, const std::string &msg);
@@ -121,3 +147,5 @@ int fx32_launch_cfl_initial(fx_domain d);
 int fx32_get_field(fx_domain d, int field, void *host);
 int fx32_set_field(fx_domain d, int field, const void *host);
 int fx32_launch_volume(fx_domain d, double *vol_out);
+int fx32_launch_ade(fx_domain d);
+int fx32_pending_weir(fx_domain d, double *vol);

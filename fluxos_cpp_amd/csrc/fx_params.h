@@ -16,6 +16,7 @@ struct fx_params {
     double arbase;            // dxy*dxy as double
     uint32_t dxy_u;           // CELLSIZE as unsigned (solver_wet's dx, dy)
     double hdry;
+    double ks0;               // interior roughness (FP32-mixed path)
     double nodata;
     double g;                 // (double)gacc
     float cfl_dxy;            // cfl*dxy evaluated in float (main.cpp:316)

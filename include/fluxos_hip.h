@@ -165,6 +165,23 @@ int fx_read_geo(int32_t nrows, int32_t ncols, double nodata, const float *dem,
 int fx_initiation(int32_t nrows, int32_t ncols, int32_t ade_solver, double *zb,
                   double *z, double *h, float *ldry, double *conc0);
 
+/* ---- kernel timing (extension; no counterpart in the reference) --------------
+ * When enabled, every kernel the domain launches is bracketed by HIP events
+ * recorded on the domain's own stream.  fx_profile_read returns the summed
+ * device time [ms] and launch count of one kernel class since the last reset. */
+#define FX_K_CFL        0   /* FP64 CFL reduction                          */
+#define FX_K_MELT       1   /* FP64 forcing + hydro loop 1                 */
+#define FX_K_FLUX       2   /* FP64 solver_dry / solver_wet                */
+#define FX_K_UPDATE     3   /* FP64 hydro loops 3+4                        */
+#define FX_K_ADE_PASS1  4   /* FP64 ADE pass 1                             */
+#define FX_K_ADE_SWEEP  5   /* FP64 ADE wavefront sweep                    */
+#define FX_K_STEP32     6   /* FP32-mixed fused step                       */
+#define FX_K_ADE32      7   /* FP32-mixed fast ADE                         */
+#define FX_K_CFL32      8   /* FP32-mixed stand-alone CFL                  */
+#define FX_K_COUNT     16
+int fx_profile(fx_domain dom, int32_t on);
+int fx_profile_read(fx_domain dom, int32_t kernel, double *total_ms, int64_t *launches);
+
 const char *fx_last_error(void);
 const char *fx_version(void);
 
