@@ -46,10 +46,18 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
 
 
+# sources of the FP32-mixed path: FP32 division and square root may be the
+# hardware's approximate forms (<= 1 ulp) and FP32 denormals (< 1.2e-38) are
+# flushed; FP64 arithmetic is unaffected
+FAST_FP32 = {"fx_fp32.hip", "fx_ade32.hip"}
+
+
 def _compile(src: Path) -> Path:
     out = OBJ / (src.name + ".o")
     if src.suffix == ".hip":
-        cmd = [HIPCC, *HIP_FLAGS, "-x", "hip", "-c", str(src), "-o", str(out)]
+        extra = (["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero"]
+                 if src.name in FAST_FP32 else [])
+        cmd = [HIPCC, *HIP_FLAGS, *extra, "-x", "hip", "-c", str(src), "-o", str(out)]
     else:
         cmd = [CXX, *CXX_FLAGS, "-I/opt/rocm/include", "-c", str(src), "-o", str(out)]
     _run(cmd)

@@ -59,6 +59,7 @@ __global__ __launch_bounds__(ANT) void k32_ade(fx_params P, a32p A, fx_ctl *ctl)
     const float area_dt = area / dt;
 
     __shared__ float s_c[ALR * ALC], s_cr[ALR * ALC], s_h[ALR * ALC], s_fe[ALR * ALC], s_fn[ALR * ALC];
+    __shared__ float s_h0[ALR * ALC], s_melt[ALR * ALC];
     __shared__ uint8_t s_w[ALR * ALC], s_wp[ALR * ALC];
     __shared__ float s_pe[APR * APC], s_pn[APR * APC];
     __shared__ float s_e[AFR * AFC], s_n[AFR * AFC];
@@ -68,21 +69,27 @@ __global__ __launch_bounds__(ANT) void k32_ade(fx_params P, a32p A, fx_ctl *ctl)
     for (int i = tid; i < ALR * ALC; i += ANT) {
         const int lr = i % ALR, lc = i / ALR;
         const int r = r0 - 3 + lr, c = c0 - 3 + lc;
-        float h = 0.f, fe = 0.f, fn = 0.f;
+        float h = 0.f, fe = 0.f, fn = 0.f, h0 = 0.f, mlt = 0.f;
         uint8_t w = 0, wp = 0;
         if (r >= 0 && r < ld && c >= 0 && c < mcols) {
             const size_t p = (size_t)r + (size_t)c * ld;
             const uint8_t f = A.flags[p];
             if (f & FXF_INNER) {
                 const double hn = fmax(0.0, d_new[p]);
+                const double dold = d_old[p];
+                const double melt = (f & FXF_NODATA) ? 0.0
+                                    : (P.melt_mult ? meteoi * (double)A.melt_mult[p] : meteoi);
                 h = (float)hn;
                 w = (!(hn < P.hdry) && !(f & FXF_WEIR)) ? 1 : 0;
-                wp = (!(fmax(0.0, d_old[p]) < P.hdry) && !(f & FXF_WEIR)) ? 1 : 0;
+                wp = (!(fmax(0.0, dold) < P.hdry) && !(f & FXF_WEIR)) ? 1 : 0;
+                h0 = (float)fmax(0.0, dold + melt);
+                mlt = dold > 0.0 ? (float)melt : 0.f;   // solute arrives with melt only into water
                 fe = A.ff[p];
                 fn = A.ff[A.mm + p];
             }
         }
         s_h[i] = h; s_fe[i] = fe; s_fn[i] = fn; s_w[i] = w; s_wp[i] = wp;
+        s_h0[i] = h0; s_melt[i] = mlt;
     }
 
     for (int k = 0; k < P.nchem; k++) {
@@ -99,20 +106,12 @@ __global__ __launch_bounds__(ANT) void k32_ade(fx_params P, a32p A, fx_ctl *ctl)
             const int lr = i % ALR, lc = i / ALR;
             const int r = r0 - 3 + lr, c = c0 - 3 + lc;
             float c1 = 0.f, cr = 0.f;
-            if (r >= 0 && r < ld && c >= 0 && c < mcols) {
+            if (s_h0[i] > 0.f || s_w[i]) {
                 const size_t p = (size_t)r + (size_t)c * ld;
-                const uint8_t f = A.flags[p];
-                if (f & FXF_INNER) {
-                    double mv = m[p];
-                    const double dold = d_old[p];
-                    const double melt = (f & FXF_NODATA) ? 0.0
-                                        : (P.melt_mult ? meteoi * (double)A.melt_mult[p] : meteoi);
-                    if (dold > 0.0) mv += melt * cm;
-                    const double h0 = fmax(0.0, dold + melt);
-                    const double craw = h0 > 0.0 ? mv / h0 : 0.0;
-                    cr = (float)craw;
-                    if (s_w[i]) c1 = s_wp[i] ? (float)(mv / (double)s_h[i]) : (float)craw;
-                }
+                const float mv = m[p] + s_melt[i] * (float)cm;
+                const float h0 = s_h0[i];
+                cr = h0 > 0.f ? mv / h0 : 0.f;
+                if (s_w[i]) c1 = s_wp[i] ? mv / s_h[i] : cr;
             }
             s_c[i] = transport ? c1 : cr;
             s_cr[i] = cr;
@@ -211,10 +210,7 @@ __global__ __launch_bounds__(ANT) void k32_ade(fx_params P, a32p A, fx_ctl *ctl)
             const int li = (lr + 3) + (lc + 3) * ALR;
             float out;
             if (!transport) {
-                double mv = m[p];
-                if (d_old[p] > 0.0 && !(A.flags[p] & FXF_NODATA))
-                    mv += (P.melt_mult ? meteoi * (double)A.melt_mult[p] : meteoi) * cm;
-                out = (float)mv;
+                out = m[p] + s_melt[li] * (float)cm;
             } else if (!s_w[li]) {
                 out = 0.f;                               // dry after the update (:69-71,119-124)
             } else {

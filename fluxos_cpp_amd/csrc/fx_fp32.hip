@@ -8,19 +8,18 @@
 // solver_drydomain.cpp:60-76) and the wet-time tracer (written only when a
 // cell changes between wet and dry).
 //
-// k32_step, one 64x8 tile per 256-thread block:
-//   1. load the tile plus a 2-cell W/S and 1-cell E/N halo into LDS, applying
-//      the melt of this step (common.cpp:210-263) and hydro loop 1
-//      (hydrodynamics_calc.cpp:53-73) on the fly;
-//   2. face fluxes of every non-weir cell of the tile and its W/S halo row and
-//      column (solver_dry / solver_wet in FP32; solver_drydomain.cpp:27-194,
-//      solver_wetdomain.cpp:27-342);
-//   3. flux divergence + update with 0.1/0.8/0.1 face smoothing
-//      (hydrodynamics_calc.cpp:93-154).  Face volume terms are FP32 products
-//      summed exactly in FP64, so every face moves the same volume out of one
-//      cell and into the other and the FP64 depth conserves water;
-//   4. CFL bound of the updated state for the next step (main.cpp:300-323),
-//      reduced per block in FP64; k32_clock folds the partials.
+// k32_sweep (one per step; described at the kernel): one wave per 63-row
+// strip walks along the columns, so every cell is read from HBM once and its
+// fluxes are computed once.  Per column it applies the melt of this step
+// (common.cpp:210-263) and hydro loop 1 (hydrodynamics_calc.cpp:53-73) as the
+// cells arrive, computes the face fluxes (solver_dry / solver_wet in FP32;
+// solver_drydomain.cpp:27-194, solver_wetdomain.cpp:27-342), updates the
+// state with the 0.1/0.8/0.1 face smoothing (hydrodynamics_calc.cpp:93-154)
+// and folds the CFL bound of the new state for the next step
+// (main.cpp:300-323) into per-block FP64 partials that k32_clock reduces.
+// Face volume terms are FP32 products summed exactly in FP64, so every face
+// moves the same volume out of one cell and into the other and the FP64
+// depth conserves water.
 // The bed is held as an FP32 high part plus an FP16 low part: differences of
 // neighbouring high parts are exact (Sterbenz) and the low parts restore the
 // FP64 bed to ~6e-8 m, so the water-surface gradient of millimetre films over
@@ -29,13 +28,6 @@
 
 #pragma clang fp contract(fast)
 
-#define TX 64
-#define TY 8
-#define LR (TX + 3)
-#define LC (TY + 3)
-#define FR (TX + 1)
-#define FC (TY + 1)
-#define NT 256
 
 namespace {
 
@@ -59,7 +51,7 @@ struct s32p {
 #define LS 8u   // stored fluxes nonzero
 
 struct consts32 {
-    float hdry, g, nuem, ks, dx;
+    float hdry, g, nuem, ks, dx, idx;
     float half_dx_over_dt, arbase_over_dt;
 };
 
@@ -110,23 +102,25 @@ __device__ inline void solver_dry32(const consts32 &K, float dp, float de, float
 }
 
 // solver_wet in FP32 (solver_wetdomain.cpp:27-342), same branch structure.
+// The eight neighbour velocities q/max(h, hdry, ks) of the eddy-viscosity
+// term (:191-200) are per-cell quantities, computed once when a cell is
+// loaded (u, v below); likewise sqrt(h) for the eddy coefficients (:188-189).
 struct nb32 {
-    float dp, de, dn, dw, ds;          // depths (may be negative)
+    float dp, de, dn, dw, ds;          // depths after melt (may be negative)
     float dzbE, dzbN;                  // bed differences E-P, N-P
-    float qp, qe, qn, qs, qse, qne;    // qx at P, E, N, S, SE, NE
-    float rp, re, rn, rw, rwn, rne;    // qy at P, E, N, W, NW, NE
-    float hse, hne, hnw;               // loop-1 depths at SE, NE, NW
+    float qp, qe, qn;                  // qx at P, E, N
+    float rp, re, rn;                  // qy at P, E, N
+    float uP, vP, uN, uS, uSE, uNE;    // cell velocities q/max(h,hdry,ks)
+    float vE, vW, vNW, vNE;
+    float sqP, sqE, sqN;               // sqrt(max(0, d))
     bool wetE, wetN;
 };
 
 __device__ inline void solver_wet32(const consts32 &K, const nb32 &n, float *f)
 {
-    const float hdry = K.hdry, g = K.g, ks = K.ks, dx = K.dx;
+    const float hdry = K.hdry, g = K.g, dx = K.dx;
     const float hp = fmaxf(0.f, n.dp);
-    const float hp0 = fmaxf(fmaxf(hdry, hp), ks);
-    const float hw = fmaxf(0.f, n.dw);
     float he = fmaxf(0.f, n.de);
-    const float hs = fmaxf(0.f, n.ds);
     float hn = fmaxf(0.f, n.dn);
     float qp = n.qp, qe = n.qe, qn = n.qn;
     float rp = n.rp, re = n.re, rn = n.rn;
@@ -172,22 +166,15 @@ __device__ inline void solver_wet32(const consts32 &K, const nb32 &n, float *f)
     const float cme = sqrtf(g * hme), cmn = sqrtf(g * hmn);
     const float ihme = 1.f / fmaxf(hme, hdry), ihmn = 1.f / fmaxf(hmn, hdry);
     const float ume = qme * ihme, vme = rme * ihme, umn = qmn * ihmn, vmn = rmn * ihmn;
-    // eddy coefficients: us is identically zero, so cvdef*us*h + nuem = nuem
-    const float hne_t = K.nuem * sqrtf(hp * he);
-    const float hnn_t = K.nuem * sqrtf(hp * hn);
-    const float ihp0 = 1.f / hp0;
-    const float up = qp * ihp0, vp = rp * ihp0;
-    const float un = qn / fmaxf(fmaxf(hn, hdry), ks);
-    const float us0 = n.qs / fmaxf(fmaxf(hs, hdry), ks);
-    const float use0 = n.qse / fmaxf(fmaxf(n.hse, hdry), ks);
-    const float une = n.qne / fmaxf(fmaxf(n.hne, hdry), ks);
-    const float ve = re / fmaxf(fmaxf(he, hdry), ks);
-    const float vw = n.rw / fmaxf(fmaxf(hw, hdry), ks);
-    const float vwn = n.rwn / fmaxf(fmaxf(n.hnw, hdry), ks);
-    const float ven = n.rne / fmaxf(fmaxf(n.hne, hdry), ks);
-    const float idx = 1.f / dx;
-    const float txye = hne_t * ((ve - vp) * idx + .25f * (un + une - us0 - use0) * idx);
-    const float txyn = hnn_t * ((un - up) * idx + .25f * (ve + ven - vw - vwn) * idx);
+    // eddy coefficients: us is identically zero, so cvdef*us*h + nuem = nuem;
+    // a dry neighbour contributes h = 0 and velocity 0 (:124-126,151-153)
+    const float hne_t = K.nuem * n.sqP * (n.wetE ? n.sqE : 0.f);
+    const float hnn_t = K.nuem * n.sqP * (n.wetN ? n.sqN : 0.f);
+    const float un = n.wetN ? n.uN : 0.f;
+    const float ve = n.wetE ? n.vE : 0.f;
+    const float idx = K.idx;
+    const float txye = hne_t * ((ve - n.vP) * idx + .25f * (un + n.uNE - n.uS - n.uSE) * idx);
+    const float txyn = hnn_t * ((un - n.uP) * idx + .25f * (ve + n.vNE - n.vW - n.vNW) * idx);
     float fe1 = qme, fe2 = qme * ume, fe3 = qme * vme - txye;
     float fn1 = rmn, fn2 = rmn * umn - txyn, fn3 = rmn * vmn;
     if (lroe) {
@@ -264,238 +251,343 @@ __device__ inline double wsum(double v)
     return v;
 }
 
-// block partials: 3 doubles per block (dt bound, max wet depth, weir-face volume)
-__device__ inline void block_partials(double dtm, double hmx, double wv, double *partials,
-                                      unsigned bid, int tid, bool write_weir)
-{
-    __shared__ double sm[3][NT / 64];
-    dtm = wmin(dtm);
-    hmx = wmax(hmx);
-    wv = wsum(wv);
-    if ((tid & 63) == 0) { sm[0][tid >> 6] = dtm; sm[1][tid >> 6] = hmx; sm[2][tid >> 6] = wv; }
-    __syncthreads();
-    if (tid == 0) {
-        double a = sm[0][0], b = sm[1][0], c = sm[2][0];
-        for (int w = 1; w < NT / 64; w++) { a = fmin(a, sm[0][w]); b = fmax(b, sm[1][w]); c += sm[2][w]; }
-        partials[3 * (size_t)bid] = a;
-        partials[3 * (size_t)bid + 1] = b;
-        if (write_weir) partials[3 * (size_t)bid + 2] = c;
-    }
-}
-
 // CFL bound of one updated cell (main.cpp:308-316), FP64
 __device__ inline void cfl_cell(const fx_params &P, double h, float qx, float qy, double &dtm, double &hmx)
 {
     if (h > P.hdry) {
         hmx = fmax(hmx, h);
-        const double c0 = sqrt(P.g * h);
-        const double u0 = fmax(1e-6, fabs((double)qx / h));
-        const double v0 = fmax(1e-6, fabs((double)qy / h));
-        dtm = fmin(dtm, fmin((double)P.cfl_dxy / (u0 + c0), (double)P.cfl_dxy / (v0 + c0)));
+        // min(a/(u0+c0), a/(v0+c0)) = a/(max(u0,v0)+c0) (division is monotonic);
+        // the bound is evaluated in FP32, the reduction over cells in FP64
+        const float hf = (float)h;
+        const float c0 = sqrtf((float)P.g * hf);
+        const float u = fmaxf(1e-6f, fmaxf(fabsf(qx), fabsf(qy)) / hf);
+        dtm = fmin(dtm, (double)(P.cfl_dxy / (u + c0)));
     }
 }
 
-__global__ __launch_bounds__(NT) void k32_step(fx_params P, s32p S, fx_ctl *ctl, int ade_on)
-{
-    if (ctl->halt) return;
-    const int cur = ctl->cur;
-    const double *d_in = S.d_in[cur];
-    double *d_out = const_cast<double *>(S.d_in[cur ^ 1]);
-    const float *qx_in = S.qx_in[cur], *qy_in = S.qy_in[cur];
-    float *qx_out = const_cast<float *>(S.qx_in[cur ^ 1]);
-    float *qy_out = const_cast<float *>(S.qy_in[cur ^ 1]);
-    const int gate = ctl->gate;
-    const double dt = ctl->dtfl;
-    const double meteoi = P.meteo ? ctl->meteoi : 0.0;
-    const int ld = P.ld, mcols = P.ncols + 2;
-    const int r0 = 1 + blockIdx.x * TX, c0 = 1 + blockIdx.y * TY;
-    const int tid = threadIdx.x + threadIdx.y * 64;
-    const unsigned bid = blockIdx.x + blockIdx.y * gridDim.x;
+// ---------------------------------------------------------------------------
+// The sweep.  Lane k of the block's single wave holds row r0-2+k of every
+// column it visits; lanes 1..62 compute fluxes (rows r0-1..r0+60) and lanes
+// 2..62 own and update rows r0..r0+60 (TH = 61).  The wave walks the columns
+// c0..c0+CH-1; per column:
+//   L(c+1): cells of column c+1 from the prefetch registers into the LDS ring
+//           (melt, loop 1, cell velocities), then prefetch column c+2;
+//   F(c):   fluxes of column c (needs columns c-1..c+1);
+//   U(c):   update of the own rows of column c (needs fluxes of c-1 and c).
+// Every lane does the same work in every phase (no divergent halo passes);
+// each cell is read from HBM once per block and only three halo rows are
+// shared with the neighbouring block.
+// ---------------------------------------------------------------------------
+#define TH 61
+#define CH 32
+#define NS 64                 // rows held per column: r0-2 .. r0+61
 
-    __shared__ float s_d[LR * LC], s_qx[LR * LC], s_qy[LR * LC], s_zb[LR * LC], s_zl[LR * LC];
-    __shared__ uint8_t s_fl[LR * LC];
-    __shared__ float s_f[6][FR * FC];
+struct pref {                 // one cell's HBM values in flight
+    double d;
+    float qx, qy, zbh, zbl, mult;
+    uint32_t f;
+};
 
-    // 1. load tile + halo, melt and loop 1 on the fly
-    for (int i = tid; i < LR * LC; i += NT) {
-        const int lr = i % LR, lc = i / LR;
-        const int r = r0 - 2 + lr, c = c0 - 2 + lc;
-        float dv = 0.f, qxv = 0.f, qyv = 0.f, zbv = 0.f, zlv = 0.f;
-        uint8_t fl = 0;
-        if (r >= 0 && r < ld && c >= 0 && c < mcols) {
-            const size_t p = (size_t)r + (size_t)c * ld;
-            const uint8_t f = S.flags[p];
-            if (f & FXF_INNER) {
-                double d = d_in[p];
-                if (!(f & FXF_NODATA)) d += P.melt_mult ? meteoi * (double)P.melt_mult[p] : meteoi;
-                const bool wet = d > P.hdry;           // h = max(0,d) > hdry
-                qxv = qx_in[p];
-                qyv = qy_in[p];
-                if (gate && !wet) { qxv = 0.f; qyv = 0.f; }
-                dv = (float)d;
-                zbv = S.zb[p];
-                zlv = (float)S.zb_lo[p];
-                fl = (wet ? LW : 0) | ((f & FXF_WEIR) ? LX : 0) | LI | ((f & FXF_STALE) ? LS : 0);
-            }
-        }
-        s_d[i] = dv; s_qx[i] = qxv; s_qy[i] = qyv; s_zb[i] = zbv; s_zl[i] = zlv; s_fl[i] = fl;
-    }
-    __syncthreads();
+struct ring {
+    float dm[3][NS], qx[3][NS], qy[3][NS], u[3][NS], v[3][NS], sq[3][NS], zbh[3][NS], zbl[3][NS];
+    uint8_t fl[3][NS];        // LW | LX | LI | LS
+    uint8_t raw[3][NS];       // flag bytes
+    double dm64[3][NS];       // FP64 depth after melt
+    float f[2][6][NS];        // fluxes
+};
 
+// per-block constants of the sweep
+struct sweep_ctx {
+    const fx_params *P;
+    const s32p *S;
+    const double *d_in;
+    double *d_out;
+    const float *qx_in, *qy_in;
+    float *qx_out, *qy_out;
     consts32 K;
-    K.hdry = (float)P.hdry;
-    K.g = (float)P.g;
-    K.nuem = (float)P.nuem;
-    K.ks = (float)P.ks0;           // uniform interior roughness (checked at fx_set_static)
-    K.dx = (float)P.dxy;
-    K.half_dx_over_dt = (float)(0.5 * P.dxy / dt);
-    K.arbase_over_dt = (float)(P.arbase / dt);
-    const float dxy_over_dt = (float)(P.dxy / dt);
+    double meteoi, s_wet_prev;
+    float dxy_over_dt, kf, dtf, hdryf;
+    int gate, ade_on, k, row, mcols;
+    bool row_ok, flux_lane, own;
+    uint32_t ld;
+};
 
-    // 2. fluxes of the tile and its W/S halo
-    if (gate) {
-        for (int i = tid; i < FR * FC; i += NT) {
-            const int fr = i % FR, fc = i / FR;
-            const int li = (fr + 1) + (fc + 1) * LR;
-            float f[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-            const uint8_t fl = s_fl[li];
-            if ((fl & LI) && !(fl & LX)) {
-                const bool wetE = s_fl[li + 1] & LW, wetN = s_fl[li + LR] & LW;
-                if (!(fl & LW)) {
-                    const bool wetW = s_fl[li - 1] & LW, wetS = s_fl[li - LR] & LW;
-                    if (!wetW && !wetE && !wetS && !wetN) {
-                        if (fl & LS) {         // stale fluxes of an all-dry neighbourhood
-                            const int r = r0 - 1 + fr, c = c0 - 1 + fc;
-                            const size_t p = (size_t)r + (size_t)c * ld;
-                            for (int k = 0; k < 6; k++) f[k] = S.f6[k * S.mm + p];
-                        }
-                    } else {
-                        // neighbour bed differences: exact high parts plus the low parts
-                        const float dzbE = (s_zb[li + 1] - s_zb[li]) + (s_zl[li + 1] - s_zl[li]);
-                        const float dzbN = (s_zb[li + LR] - s_zb[li]) + (s_zl[li + LR] - s_zl[li]);
-                        solver_dry32(K, s_d[li], s_d[li + 1], s_d[li + LR], dzbE, dzbN, wetE, wetN,
-                                     dxy_over_dt, f);
-                    }
-                } else {
-                    nb32 n;
-                    n.dp = s_d[li]; n.de = s_d[li + 1]; n.dn = s_d[li + LR];
-                    n.dw = s_d[li - 1]; n.ds = s_d[li - LR];
-                    n.dzbE = (s_zb[li + 1] - s_zb[li]) + (s_zl[li + 1] - s_zl[li]);
-                    n.dzbN = (s_zb[li + LR] - s_zb[li]) + (s_zl[li + LR] - s_zl[li]);
-                    n.qp = s_qx[li]; n.qe = s_qx[li + 1]; n.qn = s_qx[li + LR]; n.qs = s_qx[li - LR];
-                    n.qse = s_qx[li + 1 - LR]; n.qne = s_qx[li + 1 + LR];
-                    n.rp = s_qy[li]; n.re = s_qy[li + 1]; n.rn = s_qy[li + LR]; n.rw = s_qy[li - 1];
-                    n.rwn = s_qy[li - 1 + LR]; n.rne = s_qy[li + 1 + LR];
-                    n.hse = fmaxf(0.f, s_d[li + 1 - LR]);
-                    n.hne = fmaxf(0.f, s_d[li + 1 + LR]);
-                    n.hnw = fmaxf(0.f, s_d[li - 1 + LR]);
-                    n.wetE = wetE; n.wetN = wetN;
-                    solver_wet32(K, n, f);
+__device__ inline void load_col(const sweep_ctx &X, int c, pref &a)
+{
+    a.f = 0;
+    if (!X.row_ok || c < 0 || c >= X.mcols) return;      // column test is uniform
+    const uint32_t p = (uint32_t)X.row + (uint32_t)c * X.ld;
+    const s32p &S = *X.S;
+    a.f = S.flags[p];
+    a.d = X.d_in[p];
+    a.qx = X.qx_in[p];
+    a.qy = X.qy_in[p];
+    a.zbh = S.zb[p];
+    a.zbl = (float)S.zb_lo[p];
+    a.mult = X.P->melt_mult ? X.P->melt_mult[p] : 1.0f;
+}
+
+// L: this lane's cell of the column into ring slot SL (melt of this step,
+// hydro loop 1, cell velocities)
+template <int SL>
+__device__ inline void store_col(const sweep_ctx &X, ring &R, const pref &a)
+{
+    const int i = X.k;
+    const bool inner = a.f & FXF_INNER;
+    double d = inner ? a.d : 0.0;
+    if (inner && !(a.f & FXF_NODATA)) d += X.meteoi * (double)a.mult;
+    const bool wet = d > X.P->hdry;                     // loop 1: h = max(0,d) > hdry
+    float qx = inner ? a.qx : 0.f, qy = inner ? a.qy : 0.f;
+    if (X.gate && !wet) { qx = 0.f; qy = 0.f; }
+    const float dm = (float)d;
+    const float h = fmaxf(0.f, dm);
+    const float den = __builtin_amdgcn_rcpf(fmaxf(fmaxf(h, X.hdryf), X.K.ks));
+    R.dm[SL][i] = dm;
+    R.qx[SL][i] = qx;
+    R.qy[SL][i] = qy;
+    R.u[SL][i] = qx * den;
+    R.v[SL][i] = qy * den;
+    R.sq[SL][i] = sqrtf(h);
+    R.zbh[SL][i] = inner ? a.zbh : 0.f;
+    R.zbl[SL][i] = inner ? a.zbl : 0.f;
+    R.fl[SL][i] = (uint8_t)((wet ? LW : 0) | ((a.f & FXF_WEIR) ? LX : 0) | (inner ? LI : 0) |
+                            ((a.f & FXF_STALE) ? LS : 0));
+    R.raw[SL][i] = (uint8_t)a.f;
+    R.dm64[SL][i] = d;
+}
+
+// F(c): fluxes of this lane's row; column c in slot S0, c-1 in SS, c+1 in SN
+template <int S0, int SS, int SN>
+__device__ inline void flux_col(const sweep_ctx &X, ring &R, int c)
+{
+    const int i = X.k;
+    float f[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const uint8_t fl = R.fl[S0][i];
+    if (X.gate && X.flux_lane && (fl & LI) && !(fl & LX)) {
+        const bool wetE = R.fl[S0][i + 1] & LW, wetN = R.fl[SN][i] & LW;
+        const float dzbE = (R.zbh[S0][i + 1] - R.zbh[S0][i]) + (R.zbl[S0][i + 1] - R.zbl[S0][i]);
+        const float dzbN = (R.zbh[SN][i] - R.zbh[S0][i]) + (R.zbl[SN][i] - R.zbl[S0][i]);
+        if (!(fl & LW)) {
+            const bool wetW = R.fl[S0][i - 1] & LW, wetS = R.fl[SS][i] & LW;
+            if (!wetW && !wetE && !wetS && !wetN) {
+                if (fl & LS) {                     // stale fluxes of an all-dry neighbourhood
+                    const uint32_t p = (uint32_t)X.row + (uint32_t)c * X.ld;
+                    for (int m = 0; m < 6; m++) f[m] = X.S->f6[(uint32_t)m * (uint32_t)X.S->mm + p];
                 }
+            } else {
+                solver_dry32(X.K, R.dm[S0][i], R.dm[S0][i + 1], R.dm[SN][i], dzbE, dzbN, wetE, wetN,
+                             X.dxy_over_dt, f);
             }
-            for (int k = 0; k < 6; k++) s_f[k][i] = f[k];
+        } else {
+            nb32 n;
+            n.dp = R.dm[S0][i]; n.de = R.dm[S0][i + 1]; n.dn = R.dm[SN][i];
+            n.dw = R.dm[S0][i - 1]; n.ds = R.dm[SS][i];
+            n.dzbE = dzbE; n.dzbN = dzbN;
+            n.qp = R.qx[S0][i]; n.qe = R.qx[S0][i + 1]; n.qn = R.qx[SN][i];
+            n.rp = R.qy[S0][i]; n.re = R.qy[S0][i + 1]; n.rn = R.qy[SN][i];
+            n.uP = R.u[S0][i]; n.vP = R.v[S0][i];
+            n.uN = R.u[SN][i]; n.uS = R.u[SS][i];
+            n.uSE = R.u[SS][i + 1]; n.uNE = R.u[SN][i + 1];
+            n.vE = R.v[S0][i + 1]; n.vW = R.v[S0][i - 1];
+            n.vNW = R.v[SN][i - 1]; n.vNE = R.v[SN][i + 1];
+            n.sqP = R.sq[S0][i]; n.sqE = R.sq[S0][i + 1]; n.sqN = R.sq[SN][i];
+            n.wetE = wetE; n.wetN = wetN;
+            solver_wet32(X.K, n, f);
         }
     }
-    __syncthreads();
+    const int fs = c & 1;
+    for (int m = 0; m < 6; m++) R.f[fs][m][i] = f[m];
+}
 
-    // 3. update own cells, 4. CFL bound of the new state
-    double dtm = 9.e10, hmx = 0.0, wv = 0.0;
-    const float kf = (float)(dt / P.dxy);
-    const float dtf = (float)dt;
-    for (int j = 0; j < TY / 4; j++) {
-        const int lr = threadIdx.x, lc = threadIdx.y + 4 * j;
-        const int r = r0 + lr, c = c0 + lc;
-        if (r > P.nrows || c > P.ncols) continue;
-        const size_t p = (size_t)r + (size_t)c * ld;
-        const int li = (lr + 2) + (lc + 2) * LR;
-        const int fi = (lr + 1) + (lc + 1) * FR;
-        const uint8_t f = S.flags[p];
-        const bool weir = f & FXF_WEIR;
-        double d = d_in[p];
-        if (!(f & FXF_NODATA)) d += P.melt_mult ? meteoi * (double)P.melt_mult[p] : meteoi;
-        float qxn, qyn;
-        uint8_t nf = f;
-        float fe1P = 0.f, fn1P = 0.f;
-        if (!gate) {
-            qxn = qx_in[p];
-            qyn = qy_in[p];
+// U(c): hydro loops 3 and 4 for this lane's own cell, and its CFL bound
+template <int S0, int SS, int SN>
+__device__ inline void update_col(const sweep_ctx &X, ring &R, int c, double &dtm, double &hmx,
+                                  double &wv)
+{
+    if (!X.own) return;
+    const fx_params &P = *X.P;
+    const s32p &S = *X.S;
+    const int i = X.k, fs = c & 1, fsS = (c - 1) & 1;
+    const uint32_t p = (uint32_t)X.row + (uint32_t)c * X.ld;
+    const uint8_t f = R.raw[S0][i];
+    const uint8_t fl = R.fl[S0][i];
+    const bool weir = f & FXF_WEIR;
+    double d = R.dm64[S0][i];
+    float qxn, qyn;
+    uint8_t nf = f;
+    float fe1P = 0.f, fn1P = 0.f;
+    if (!X.gate) {
+        qxn = R.qx[S0][i];
+        qyn = R.qy[S0][i];
+    } else {
+        const float kf = X.kf, dtf = X.dtf;
+        const float fe1W = R.f[fs][0][i - 1], fn1S = R.f[fsS][3][i];
+        fe1P = R.f[fs][0][i];
+        fn1P = R.f[fs][3][i];
+        float dqx = 0.f, dqy = 0.f;
+        if (!weir) {
+            const float a1 = fe1W * kf, b1 = fe1P * kf, c1 = fn1S * kf, e1 = fn1P * kf;
+            d += ((double)a1 - (double)b1) + ((double)c1 - (double)e1);
+            dqx = ((R.f[fs][1][i - 1] - R.f[fs][1][i]) + (R.f[fsS][4][i] - R.f[fs][4][i])) * kf;
+            dqy = ((R.f[fs][2][i - 1] - R.f[fs][2][i]) + (R.f[fsS][5][i] - R.f[fs][5][i])) * kf;
+            if (R.fl[S0][i + 1] & LX) wv -= (double)b1;        // into / out of a weir cell
+            if (R.fl[SN][i] & LX) wv -= (double)e1;
+        }
+        const double hd = fmax(0.0, d);
+        if (hd < P.hdry || weir) {
+            qxn = 0.f;
+            qyn = 0.f;
         } else {
-            const float fe1W = s_f[0][fi - 1], fn1S = s_f[3][fi - FR];
-            fe1P = s_f[0][fi];
-            fn1P = s_f[3][fi];
-            float dqx = 0.f, dqy = 0.f;
-            if (!weir) {
-                const float a = fe1W * kf, b = fe1P * kf, cc = fn1S * kf, e = fn1P * kf;
-                d += ((double)a - (double)b) + ((double)cc - (double)e);
-                dqx = ((s_f[1][fi - 1] - s_f[1][fi]) + (s_f[4][fi - FR] - s_f[4][fi])) * kf;
-                dqy = ((s_f[2][fi - 1] - s_f[2][fi]) + (s_f[5][fi - FR] - s_f[5][fi])) * kf;
-                if (s_fl[li + 1] & LX) wv -= (double)b;        // into / out of a weir cell
-                if (s_fl[li + LR] & LX) wv -= (double)e;
-            }
-            const double h = fmax(0.0, d);
-            if (h < P.hdry || weir) {
-                qxn = 0.f;
-                qyn = 0.f;
-            } else {
-                qxn = .1f * (fe1W * dtf) + .8f * (s_qx[li] + dqx) + .1f * (fe1P * dtf);
-                qyn = .1f * (fn1S * dtf) + .8f * (s_qy[li] + dqy) + .1f * (fn1P * dtf);
-            }
-            // keep the fluxes of a cell that may be all-dry next step (S1)
-            if (!weir) {
-                const bool wet1 = s_fl[li] & LW;
-                const bool alldry = !wet1 && !(s_fl[li - 1] & LW) && !(s_fl[li + 1] & LW) &&
-                                    !(s_fl[li - LR] & LW) && !(s_fl[li + LR] & LW);
-                if (!alldry && (h <= P.hdry || !P.rates_nonneg)) {
-                    bool nz = false;
-                    for (int k = 0; k < 6; k++) nz |= s_f[k][fi] != 0.f;
-                    if (nz) {
-                        for (int k = 0; k < 6; k++) S.f6[k * S.mm + p] = s_f[k][fi];
-                        nf |= FXF_STALE;
-                    } else {
-                        nf &= ~FXF_STALE;
-                    }
+            qxn = .1f * (fe1W * dtf) + .8f * (R.qx[S0][i] + dqx) + .1f * (fe1P * dtf);
+            qyn = .1f * (fn1S * dtf) + .8f * (R.qy[S0][i] + dqy) + .1f * (fn1P * dtf);
+        }
+        // keep the fluxes of a cell that may be all-dry next step (S1)
+        const bool wet1 = fl & LW;
+        if (!weir && (hd <= P.hdry || !P.rates_nonneg)) {
+            const bool alldry = !wet1 && !(R.fl[S0][i - 1] & LW) && !(R.fl[S0][i + 1] & LW) &&
+                                !(R.fl[SS][i] & LW) && !(R.fl[SN][i] & LW);
+            if (!alldry) {
+                bool nz = false;
+                for (int m = 0; m < 6; m++) nz |= R.f[fs][m][i] != 0.f;
+                if (nz) {
+                    for (int m = 0; m < 6; m++) S.f6[(uint32_t)m * (uint32_t)S.mm + p] = R.f[fs][m][i];
+                    nf |= FXF_STALE;
+                } else {
+                    nf &= ~FXF_STALE;
                 }
             }
-            // wet-time tracer: close / open runs on transitions only
-            const bool wet1 = s_fl[li] & LW;
-            if (wet1 && !(f & FXF_WETPREV)) {
-                S.tw_t0[p] = (float)ctl->s_wet_prev;
+        }
+        // wet-time tracer: runs are opened / closed on transitions only
+        if (wet1 != (bool)(f & FXF_WETPREV)) {
+            if (wet1) {
+                S.tw_t0[p] = (float)X.s_wet_prev;
                 nf |= FXF_WETPREV;
-            } else if (!wet1 && (f & FXF_WETPREV)) {
-                S.tw_acc[p] += (float)(ctl->s_wet_prev - (double)S.tw_t0[p]);
+            } else {
+                S.tw_acc[p] += (float)(X.s_wet_prev - (double)S.tw_t0[p]);
                 nf &= ~FXF_WETPREV;
             }
         }
-        d_out[p] = d;
-        qx_out[p] = qxn;
-        qy_out[p] = qyn;
-        if (ade_on) {
-            S.ff[p] = fe1P;
-            S.ff[S.mm + p] = fn1P;
-        }
-        if (nf != f) S.flags[p] = nf;
-        cfl_cell(P, fmax(0.0, d), qxn, qyn, dtm, hmx);
     }
-    block_partials(dtm, hmx, wv, S.partials, bid, tid, true);
+    X.d_out[p] = d;
+    X.qx_out[p] = qxn;
+    X.qy_out[p] = qyn;
+    if (X.ade_on) {
+        S.ff[p] = fe1P;
+        S.ff[(uint32_t)S.mm + p] = fn1P;
+    }
+    if (nf != f) S.flags[p] = nf;
+    cfl_cell(P, fmax(0.0, d), qxn, qyn, dtm, hmx);
 }
 
-// CFL partials of the current state (start of a device-resident run)
-__global__ __launch_bounds__(NT) void k32_cfl(fx_params P, s32p S, fx_ctl *ctl)
+// one column of the sweep; SC = ring slot of column c
+template <int SC>
+__device__ inline void column_step(const sweep_ctx &X, ring &R, int c, int c_last, pref &a,
+                                   double &dtm, double &hmx, double &wv)
+{
+    constexpr int SN = (SC + 1) % 3, SS = (SC + 2) % 3;
+    __syncthreads();
+    store_col<SN>(X, R, a);                           // column c+1 (prefetched last step)
+    if (c + 2 <= c_last + 1) load_col(X, c + 2, a);
+    __syncthreads();
+    flux_col<SC, SS, SN>(X, R, c);
+    __syncthreads();
+    update_col<SC, SS, SN>(X, R, c, dtm, hmx, wv);
+}
+
+__global__ __launch_bounds__(64) void k32_sweep(fx_params P, s32p S, fx_ctl *ctl, int ade_on)
+{
+    if (ctl->halt) return;
+    __shared__ ring R;
+    sweep_ctx X;
+    const int cur = ctl->cur;
+    X.P = &P;
+    X.S = &S;
+    X.d_in = S.d_in[cur];
+    X.d_out = const_cast<double *>(S.d_in[cur ^ 1]);
+    X.qx_in = S.qx_in[cur];
+    X.qy_in = S.qy_in[cur];
+    X.qx_out = const_cast<float *>(S.qx_in[cur ^ 1]);
+    X.qy_out = const_cast<float *>(S.qy_in[cur ^ 1]);
+    X.gate = ctl->gate;
+    X.ade_on = ade_on;
+    X.meteoi = P.meteo ? ctl->meteoi : 0.0;
+    X.s_wet_prev = ctl->s_wet_prev;
+    X.k = threadIdx.x;
+    const int r0 = 1 + blockIdx.x * TH;
+    const int c0 = 1 + blockIdx.y * CH;
+    const int c_last = min(c0 + CH - 1, P.ncols);
+    X.row = r0 - 2 + X.k;                             // this lane's row
+    X.row_ok = X.row >= 0 && X.row < P.ld;
+    X.flux_lane = X.k >= 1 && X.k <= NS - 2;
+    X.own = X.k >= 2 && X.k <= NS - 2 && X.row <= P.nrows;
+    X.mcols = P.ncols + 2;
+    X.ld = (uint32_t)P.ld;
+    X.hdryf = (float)P.hdry;
+    X.K.hdry = (float)P.hdry;
+    X.K.g = (float)P.g;
+    X.K.nuem = (float)P.nuem;
+    X.K.ks = (float)P.ks0;
+    X.K.dx = (float)P.dxy;
+    X.K.idx = 1.f / X.K.dx;
+    X.K.half_dx_over_dt = ctl->k_half_dx_dt;
+    X.K.arbase_over_dt = ctl->k_arbase_dt;
+    X.dxy_over_dt = ctl->k_dxy_dt;
+    X.kf = ctl->k_dt_dxy;
+    X.dtf = ctl->k_dt;
+
+    pref a;
+    double dtm = 9.e10, hmx = 0.0, wv = 0.0;
+    // prologue: columns c0-2 (slot 1), c0-1 (slot 2), c0 (slot 0), fluxes of c0-1
+    load_col(X, c0 - 2, a); store_col<1>(X, R, a);
+    load_col(X, c0 - 1, a); store_col<2>(X, R, a);
+    load_col(X, c0, a); store_col<0>(X, R, a);
+    load_col(X, c0 + 1, a);
+    __syncthreads();
+    flux_col<2, 1, 0>(X, R, c0 - 1);
+    // column c uses ring slot (c - c0) % 3, unrolled by three
+    int c = c0;
+    for (; c + 2 <= c_last; c += 3) {
+        column_step<0>(X, R, c, c_last, a, dtm, hmx, wv);
+        column_step<1>(X, R, c + 1, c_last, a, dtm, hmx, wv);
+        column_step<2>(X, R, c + 2, c_last, a, dtm, hmx, wv);
+    }
+    if (c <= c_last) { column_step<0>(X, R, c, c_last, a, dtm, hmx, wv); c++; }
+    if (c <= c_last) { column_step<1>(X, R, c, c_last, a, dtm, hmx, wv); c++; }
+    dtm = wmin(dtm);
+    hmx = wmax(hmx);
+    wv = wsum(wv);
+    if (X.k == 0) {
+        const size_t bid = blockIdx.x + (size_t)blockIdx.y * gridDim.x;
+        S.partials[3 * bid] = dtm;
+        S.partials[3 * bid + 1] = hmx;
+        S.partials[3 * bid + 2] = wv;
+    }
+}
+
+// CFL partials of the current state (start of a device-resident run), same
+// tiling as k32_sweep so the partials line up
+__global__ __launch_bounds__(64) void k32_cfl(fx_params P, s32p S, fx_ctl *ctl)
 {
     const int cur = ctl->cur;
     const double *d_in = S.d_in[cur];
     const float *qx_in = S.qx_in[cur], *qy_in = S.qy_in[cur];
-    const int r0 = 1 + blockIdx.x * TX, c0 = 1 + blockIdx.y * TY;
-    const int tid = threadIdx.x + threadIdx.y * 64;
-    const unsigned bid = blockIdx.x + blockIdx.y * gridDim.x;
+    const int k = threadIdx.x;
+    const int r0 = 1 + blockIdx.x * TH, c0 = 1 + blockIdx.y * CH;
+    const int row = r0 - 2 + k;
     double dtm = 9.e10, hmx = 0.0;
-    for (int j = 0; j < TY / 4; j++) {
-        const int r = r0 + threadIdx.x, c = c0 + threadIdx.y + 4 * j;
-        if (r > P.nrows || c > P.ncols) continue;
-        const size_t p = (size_t)r + (size_t)c * P.ld;
-        cfl_cell(P, fmax(0.0, d_in[p]), qx_in[p], qy_in[p], dtm, hmx);
+    if (k >= 2 && k <= NS - 2 && row <= P.nrows) {
+        for (int c = c0; c <= min(c0 + CH - 1, P.ncols); c++) {
+            const uint32_t p = (uint32_t)row + (uint32_t)c * (uint32_t)P.ld;
+            cfl_cell(P, fmax(0.0, d_in[p]), qx_in[p], qy_in[p], dtm, hmx);
+        }
     }
-    block_partials(dtm, hmx, 0.0, S.partials, bid, tid, false);
+    dtm = wmin(dtm);
+    hmx = wmax(hmx);
+    if (k == 0) {
+        const size_t bid = blockIdx.x + (size_t)blockIdx.y * gridDim.x;
+        S.partials[3 * bid] = dtm;
+        S.partials[3 * bid + 1] = hmx;
+    }
 }
 
 // Clock of one loop iteration (main.cpp:271-325, 385-387): fold the CFL
@@ -539,6 +631,11 @@ __global__ __launch_bounds__(1024) void k32_clock(fx_params P, double *partials,
     ctl->it += gate;
     ctl->steps += 1;
     ctl->solver_steps += gate;
+    ctl->k_dt_dxy = (float)(dt / P.dxy);
+    ctl->k_dt = (float)dt;
+    ctl->k_half_dx_dt = (float)(0.5 * P.dxy / dt);
+    ctl->k_arbase_dt = (float)(P.arbase / dt);
+    ctl->k_dxy_dt = (float)(P.dxy / dt);
     ctl->s_wet_prev = ctl->s_wet;
     if (gate) ctl->s_wet = ctl->s_wet + dt / 3600;
     fx_forcing_eval(P, ctl);
@@ -557,32 +654,26 @@ __global__ void k32_flip(fx_ctl *ctl)
 }
 
 // water volume of non-weir interior cells (FP64, block partials)
-__global__ __launch_bounds__(NT) void k32_volume(fx_params P, s32p S, fx_ctl *ctl, double *part)
+__global__ __launch_bounds__(64) void k32_volume(fx_params P, s32p S, fx_ctl *ctl, double *part)
 {
     const double *d_in = S.d_in[ctl->cur];
-    const int r0 = 1 + blockIdx.x * TX, c0 = 1 + blockIdx.y * TY;
-    const int tid = threadIdx.x + threadIdx.y * 64;
+    const int k = threadIdx.x;
+    const int r0 = 1 + blockIdx.x * TH, c0 = 1 + blockIdx.y * CH;
+    const int row = r0 - 2 + k;
     double v = 0.0;
-    for (int j = 0; j < TY / 4; j++) {
-        const int r = r0 + threadIdx.x, c = c0 + threadIdx.y + 4 * j;
-        if (r > P.nrows || c > P.ncols) continue;
-        const size_t p = (size_t)r + (size_t)c * P.ld;
-        if (!(S.flags[p] & FXF_WEIR)) v += d_in[p];
+    if (k >= 2 && k <= NS - 2 && row <= P.nrows) {
+        for (int c = c0; c <= min(c0 + CH - 1, P.ncols); c++) {
+            const uint32_t p = (uint32_t)row + (uint32_t)c * (uint32_t)P.ld;
+            if (!(S.flags[p] & FXF_WEIR)) v += d_in[p];
+        }
     }
     v = wsum(v);
-    __shared__ double sm[NT / 64];
-    if ((tid & 63) == 0) sm[tid >> 6] = v;
-    __syncthreads();
-    if (tid == 0) {
-        double s = 0.0;
-        for (int w = 0; w < NT / 64; w++) s += sm[w];
-        part[blockIdx.x + blockIdx.y * gridDim.x] = s;
-    }
+    if (k == 0) part[blockIdx.x + (size_t)blockIdx.y * gridDim.x] = v;
 }
 
 inline dim3 grid32(const fx_params &P)
 {
-    return dim3((P.nrows + TX - 1) / TX, (P.ncols + TY - 1) / TY);
+    return dim3((P.nrows + TH - 1) / TH, (P.ncols + CH - 1) / CH);
 }
 
 s32p make_s32p(fx_domain d)
@@ -864,7 +955,7 @@ int fx32_launch_cfl_initial(fx_domain d)
         d->melt_area_ok = true;
     }
     fx_prof_begin(d, FX_K_CFL32);
-    hipLaunchKernelGGL(k32_cfl, grid32(P), dim3(64, 4), 0, d->stream, P, S, d->ctl);
+    hipLaunchKernelGGL(k32_cfl, grid32(P), dim3(64), 0, d->stream, P, S, d->ctl);
     fx_prof_end(d, FX_K_CFL32);
     return fx_hip_check(hipGetLastError(), "k32_cfl");
 }
@@ -878,7 +969,7 @@ int fx32_launch_step(fx_domain d)
     FX_HIP(hipGetLastError());
     const int ade_on = (d->desc.ade_solver && d->desc.ade_mode == FX_ADE_FAST) ? 1 : 0;
     fx_prof_begin(d, FX_K_STEP32);
-    hipLaunchKernelGGL(k32_step, grid32(P), dim3(64, 4), 0, d->stream, P, S, d->ctl, ade_on);
+    hipLaunchKernelGGL(k32_sweep, grid32(P), dim3(64), 0, d->stream, P, S, d->ctl, ade_on);
     fx_prof_end(d, FX_K_STEP32);
     FX_HIP(hipGetLastError());
     if (ade_on) {
@@ -897,7 +988,7 @@ int fx32_launch_volume(fx_domain d, double *vol_out)
     const size_t nb = (size_t)g.x * g.y;
     double *part = nullptr;
     FX_HIP(hipMalloc((void **)&part, nb * sizeof(double)));
-    hipLaunchKernelGGL(k32_volume, g, dim3(64, 4), 0, d->stream, P, S, d->ctl, part);
+    hipLaunchKernelGGL(k32_volume, g, dim3(64), 0, d->stream, P, S, d->ctl, part);
     std::vector<double> h(nb);
     FX_HIP(hipMemcpyAsync(h.data(), part, nb * sizeof(double), hipMemcpyDeviceToHost, d->stream));
     FX_HIP(hipStreamSynchronize(d->stream));

@@ -43,6 +43,8 @@ struct fx_ctl {
     double s_wet;        // wet-time prefix sum [h] over solver steps
     double s_wet_prev;
     double melt_area;    // sum of the melt multiplier over non-weir bed cells
+    // per-step FP32 constants of the FP32-mixed kernels (set by k32_clock)
+    float k_dt_dxy, k_dt, k_half_dx_dt, k_arbase_dt, k_dxy_dt, k_pad;
 };
 
 // launch mode bits

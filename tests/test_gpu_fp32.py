@@ -97,15 +97,20 @@ def test_fp32_mass_balance():
 
 def test_fp32_ade_fast_bounds():
     """Fast ADE: finite, within the physical bounds of the inputs (melt at
-    1 mg/L into solute-free water), close to the exact sweep over a short span."""
+    1 mg/L into solute-free water), and close to the exact sweep over a span
+    in which the two hydro paths still agree pointwise (cells whose wet/dry
+    history differs carry the reference's reset-to-zero and are excluded by
+    comparing on cells wet in both)."""
     sc = gen.scenario_c1(64, seed=1)
     _, d64, d32 = _pair(sc, ade=True)
     for d in (d64, d32):
-        _advance_to(d, 720.0)
+        _advance_to(d, 690.0)
     c64, c32 = d64.get_conc(0)[1:-1, 1:-1], d32.get_conc(0)[1:-1, 1:-1]
     wet = (d64.get("ldry")[1:-1, 1:-1] == 0) & (d32.get("ldry")[1:-1, 1:-1] == 0)
     assert np.isfinite(c32).all()
     assert c32.min() >= -1e-6 and c32.max() <= 1.0 + 1e-4
-    diff = float(np.abs(c64 - c32)[wet].max()) if wet.any() else 0.0
-    print(f"ADE fast vs exact after ~60 solver steps: max |dc| on wet cells = {diff:.3e} mg/L", flush=True)
-    assert diff < 0.05
+    dc = np.abs(c64 - c32)[wet]
+    print(f"ADE fast vs exact at t=690 s: max |dc| {dc.max():.3e}, p99 {np.percentile(dc, 99):.3e}, "
+          f"mean {dc.mean():.3e} mg/L over {wet.sum()} wet cells", flush=True)
+    assert np.percentile(dc, 99) < 0.01
+    assert dc.max() < 0.05
