@@ -13,6 +13,8 @@
 //                  + solver_wetdomain.cpp:27-342
 //   k64_update  <- hydrodynamics_calc.cpp:93-154 (loops 3 and 4 fused: qxf/qyf
 //                  are recomputed from the neighbour's fe_1/fn_1)
+#include <algorithm>
+
 #include "fx_params.h"
 
 #define BX 64
@@ -53,35 +55,38 @@ __global__ __launch_bounds__(256) void k64_cfl(fx_params P, fx_state64 s, fx_ctl
                                                double *partials, unsigned mode)
 {
     if ((mode & FXM_GATED) && ctl->halt) return;
-    int irow, icol;
+    // grid-stride over columns (a bounded number of blocks keeps the
+    // last-block-done counter cheap), threads along the contiguous irow axis
     double dtmin = 9.e10, hmax = 0.0;
-    if (fx_interior(P, irow, icol)) {
-        const size_t p = (size_t)irow + (size_t)icol * P.ld;
-        const double hc = s.h[p];
-        s.h0[p] = hc;
-        s.ldry_prev[p] = s.ldry[p];
-        if (hc > P.hdry) {
-            s.ldry[p] = 0;
-            const double hp = fmax(hc, P.hdry);
-            hmax = hc;
-            const double c0 = sqrt(P.g * hc);
-            const double u0 = fmax(.000001, fabs(s.qx[p] / hp));
-            const double v0 = fmax(.000001, fabs(s.qy[p] / hp));
-            dtmin = fmin((double)P.cfl_dxy / (u0 + c0), (double)P.cfl_dxy / (v0 + c0));
-        } else {
-            s.ldry[p] = 1;
+    const int tid = threadIdx.x;
+    for (int icol = 1 + blockIdx.x; icol <= P.ncols; icol += gridDim.x) {
+        for (int irow = 1 + tid; irow <= P.nrows; irow += 256) {
+            const size_t p = (size_t)irow + (size_t)icol * P.ld;
+            const double hc = s.h[p];
+            s.h0[p] = hc;
+            s.ldry_prev[p] = s.ldry[p];
+            if (hc > P.hdry) {
+                s.ldry[p] = 0;
+                const double hp = fmax(hc, P.hdry);
+                hmax = fmax(hmax, hc);
+                const double c0 = sqrt(P.g * hc);
+                const double u0 = fmax(.000001, fabs(s.qx[p] / hp));
+                const double v0 = fmax(.000001, fabs(s.qy[p] / hp));
+                dtmin = fmin(fmin((double)P.cfl_dxy / (u0 + c0), (double)P.cfl_dxy / (v0 + c0)), dtmin);
+            } else {
+                s.ldry[p] = 1;
+            }
         }
     }
     // block reduction
     __shared__ double sm_dt[256 / 64], sm_h[256 / 64];
     __shared__ bool am_last;
-    const int tid = threadIdx.x + threadIdx.y * BX;
     dtmin = wave_min(dtmin);
     hmax = wave_max(hmax);
     if ((tid & 63) == 0) { sm_dt[tid >> 6] = dtmin; sm_h[tid >> 6] = hmax; }
     __syncthreads();
-    const unsigned nblk = gridDim.x * gridDim.y;
-    const unsigned bid = blockIdx.x + blockIdx.y * gridDim.x;
+    const unsigned nblk = gridDim.x;
+    const unsigned bid = blockIdx.x;
     if (tid == 0) {
         double a = sm_dt[0], b = sm_h[0];
         for (int w = 1; w < 4; w++) { a = fmin(a, sm_dt[w]); b = fmax(b, sm_h[w]); }
@@ -518,8 +523,9 @@ inline dim3 grid_of(const fx_params &P)
 int fx64_launch_cfl(fx_domain d, unsigned mode)
 {
     fx_params P = fx_make_params(d);
+    const int nb = std::min(std::min(P.ncols, 2048), d->npartials);
     fx_prof_begin(d, FX_K_CFL);
-    hipLaunchKernelGGL(k64_cfl, grid_of(P), dim3(BX, BY), 0, d->stream, P, d->s64, d->ctl,
+    hipLaunchKernelGGL(k64_cfl, dim3(nb), dim3(256), 0, d->stream, P, d->s64, d->ctl,
                        d->partials, mode);
     fx_prof_end(d, FX_K_CFL);
     return fx_hip_check(hipGetLastError(), "k64_cfl");

@@ -1,0 +1,411 @@
+"""Pins of the CPU oracle that do not restate its formulas.
+
+The reference ships no tests, fixtures or golden vectors, and it cannot be
+built here (Armadillo is absent), so the oracle is pinned against what the
+physics and the reference's documented behaviour fix independently:
+
+* CFL: dt = cfl*dxy/(max(|u|,|v|,1e-6) + sqrt(g h)) -- textbook bound;
+* forcing: the "first row later than t" lookup, unit conversion mm/day -> m;
+* a dry basin only accumulates melt (no fluxes exist);
+* lake at rest: zero mass flux, and the only momentum change is the
+  hydrostatic thrust 1/2 g H^2 against the W/S walls (whose faces carry no
+  flux) -- times dt/dxy, then the 0.8 centre weight of the face smoothing;
+* Ritter's dam-break solution q = 8/27 h sqrt(g h), depth 4/9 h at the
+  front, for a lone wet cell and for a dry cell next to a wet one;
+* Roe's upwind mass flux -1/2 c dz across a still free-surface step, and
+  the eddy-viscosity shear stress nu H dv/dx in a linear shear flow;
+* conservation: water in a walled basin, solute under pure transport;
+* diffusion: one ADE sweep with still water is the explicit 5-point scheme
+  c + 2 D dt / dx^2 * Laplacian(c) (sigc = 0.5), exact on a quadratic;
+* read_geo + initiation on hand-worked DEMs, including the two quirks the
+  reference has (the corner with all-NODATA vertices becomes NaN, and the
+  first-column copy taken before the ghost fill, initiate.cpp:44-51).
+
+tests/test_oracle_plants.py rebuilds the oracle with planted mistakes and
+shows each of these checks fails on its mistake.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleDomain
+
+G = float(np.float32(9.80665))          # gacc is a float (GlobVar.h:78)
+
+
+def flat_domain(n=12, dxy=2, h=0.0, hdry=1e-4, ade=True, nchem=1, lib=None, bed=500.0,
+                ring_bed=None):
+    """n x n interior, weir ring, flat bed, depth h everywhere (ring included)."""
+    d = OracleDomain(n, n, dxy, hdry, nchem, [0.01] * nchem, ade, lib_path=lib)
+    d.zb[...] = bed
+    d.weir[...] = 0.0
+    d.weir[1, 1:-1] = d.weir[-2, 1:-1] = d.weir[1:-1, 1] = d.weir[1:-1, -2] = 1.0
+    if ring_bed is not None:
+        ring = d.weir == 1.0
+        d.zb[ring] = ring_bed
+    d.ks[1:-1, 1:-1] = hdry
+    d.z[...] = d.zb + h
+    d.h[...] = np.maximum(d.z - d.zb, 0.0)
+    d.ldry[1:-1, 1:-1] = np.where(d.h[1:-1, 1:-1] > hdry, 0.0, 1.0)
+    return d
+
+
+# ---------------------------------------------------------------- CFL ----
+def check_cfl(lib=None):
+    d = flat_domain(h=0.3, lib=lib)
+    d.qx[5, 6] = 0.12
+    d.qy[7, 4] = -0.21
+    d.tim = 100.0
+    hp = d.courant(print_next=1e6)
+    dt_still = 2.0 / (1e-6 + math.sqrt(G * 0.3))
+    dt_u = 2.0 / (0.12 / 0.3 + math.sqrt(G * 0.3))
+    dt_v = 2.0 / (0.21 / 0.3 + math.sqrt(G * 0.3))
+    assert hp == pytest.approx(0.3, rel=1e-14)
+    assert d.dtfl == pytest.approx(min(dt_still, dt_u, dt_v), rel=1e-14)
+    assert d.tim == pytest.approx(100.0 + d.dtfl, rel=1e-15)
+    # print clip (main.cpp:321)
+    d2 = flat_domain(h=0.3, lib=lib)
+    d2.tim = 599.5
+    d2.courant(print_next=600.0)
+    assert d2.dtfl == 0.5 and d2.tim == 600.0
+
+
+def test_cfl_closed_form():
+    check_cfl()
+
+
+def test_cfl_flags_and_copies():
+    d = flat_domain(h=0.0)
+    d.h[3, 3] = 0.05
+    d.ldry[3, 3] = 1.0
+    hp = d.courant(600.0)
+    assert hp == 0.05
+    assert d.ldry[3, 3] == 0.0 and d.ldry_prev[3, 3] == 1.0 and d.h0[3, 3] == 0.05
+    assert (d.ldry[1:-1, 1:-1].sum()) == d.nrows * d.ncols - 1
+
+
+# ------------------------------------------------------------ forcing ----
+def check_forcing(lib=None):
+    d = flat_domain(h=0.0, lib=lib)
+    tab = np.array([[0, 0, 0], [0, 10, 1], [600, 20, 2], [1200, 30, 3]], dtype=np.float32)
+    d.set_meteo(tab)
+    d.zb[4, 4] = -99999.0                      # a NODATA cell receives nothing
+    z0 = d.z.copy()
+    d.tim, d.dtfl = 600.0, 60.0                # exactly on a row time: the NEXT row applies
+    d.add_meteo()
+    melt = 30.0 / (1000.0 * 3600.0 * 24.0) * 60.0
+    inner = np.zeros_like(d.z, dtype=bool)
+    inner[1:-1, 1:-1] = True
+    inner[4, 4] = False
+    assert np.allclose(d.z[inner] - z0[inner], melt, rtol=0, atol=1e-13)
+    assert d.z[4, 4] == z0[4, 4]
+    # past the end of the table: no melt
+    d.tim = 5000.0
+    z1 = d.z.copy()
+    d.add_meteo()
+    assert np.array_equal(d.z, z1)
+
+
+def test_forcing_lookup_and_units():
+    check_forcing()
+
+
+def test_melt_mixing_of_concentration():
+    d = flat_domain(h=0.0)
+    d.set_meteo(np.array([[0, 0, 0], [3600, 86.4, 5.0]], dtype=np.float32))
+    d.z[5, 5] += 0.002                        # one cell already holds 2 mm at c = 1
+    d.conc[0][5, 5] = 1.0
+    d.tim, d.dtfl = 10.0, 1000.0
+    d.add_meteo()
+    m = float(np.float32(86.4)) / 86.4e6 * 1000.0   # ~1 mm; the table is FP32 (GlobVar.h:72-73)
+    h = 0.002 + m
+    assert d.conc[0][5, 5] == pytest.approx((1.0 * 0.002 + m * 5.0) / h, rel=1e-12)
+    # a cell that was dry gets water but no solute (common.cpp:226)
+    assert d.conc[0][6, 6] == 0.0
+
+
+def test_dry_basin_only_accumulates_melt():
+    """No cell ever exceeds hdry: the solver never runs, dt is the print step,
+    and the surface rises by exactly the melt."""
+    d = flat_domain(h=0.0, hdry=1.0)
+    d.set_meteo(np.array([[0, 0, 0], [3600, 10, 1], [7200, 40, 1]], dtype=np.float32))
+    z0 = d.z.copy()
+    pn = 600.0
+    total = 0.0
+    while d.tim <= 7200:
+        flags, hp = d.step(pn)
+        assert flags & 1 == 0 and hp == 0.0 and d.dtfl == pytest.approx(600.0)
+        total += (10.0 if d.tim < 3600 else 40.0 if d.tim < 7200 else 0.0) / 86.4e6 * 600.0
+        if d.tim >= pn:
+            pn += 600.0
+    assert np.allclose(d.z[1:-1, 1:-1] - z0[1:-1, 1:-1], total, atol=1e-12)
+    assert np.count_nonzero(d.fe1) == 0
+
+
+# -------------------------------------------------------- lake at rest ----
+def check_lake_at_rest(lib=None):
+    H = 0.25
+    d = flat_domain(n=10, h=H, lib=lib)
+    d.dtfl = 0.3
+    d.hydrodynamics()
+    dt, dxy = 0.3, 2.0
+    assert np.array_equal(d.z, flat_domain(n=10, h=H).z), "still water must not move"
+    thrust = -(0.5 * G * H * H) / dxy * dt * 0.8
+    qx, qy = d.qx[1:-1, 1:-1], d.qy[1:-1, 1:-1]
+    inner = slice(1, -1)          # non-weir rows/cols 2..n-1 -> indices 1..n-2 of the interior
+    # first non-weir row (irow = 2) feels the wall to its west; every other cell is balanced
+    assert np.allclose(qx[1, 1:-1], thrust, rtol=1e-12, atol=0)
+    assert np.allclose(qx[2:-1, 1:-1], 0.0, atol=1e-15)
+    assert np.allclose(qy[1:-1, 1], thrust, rtol=1e-12, atol=0)
+    assert np.allclose(qy[1:-1, 2:-1], 0.0, atol=1e-15)
+    assert np.count_nonzero(d.fe1) == 0 and np.count_nonzero(d.fn1) == 0
+
+
+def test_lake_at_rest_wall_thrust():
+    check_lake_at_rest()
+
+
+# -------------------------------------------------------------- Ritter ----
+def check_ritter(lib=None):
+    H, dt, dxy = 0.1, 0.5, 2.0
+    d = flat_domain(n=10, h=0.0, lib=lib)
+    d.z[5, 5] += H
+    d.h[5, 5] = H
+    d.dtfl = dt
+    d.hydrodynamics()
+    q = 0.296 * H * math.sqrt(G * H)               # 8/27 h sqrt(g h) to 3 digits
+    assert 0.296 == pytest.approx(8 / 27, abs=5e-4)
+    assert d.fe1[5, 5] == pytest.approx(q, rel=1e-13)
+    assert d.fn1[5, 5] == pytest.approx(q, rel=1e-13)
+    # the dry cell west of the wet one draws the same Ritter discharge
+    # inward, with depth 4/9 H at the front (solver_drydomain.cpp:127-138)
+    hf = 0.444 * H
+    assert d.fe1[4, 5] == pytest.approx(-q, rel=1e-13)
+    assert d.fe2[4, 5] == pytest.approx(q * q / hf + 0.5 * G * hf * hf, rel=1e-12)
+    assert d.fn1[5, 4] == pytest.approx(-q, rel=1e-13)
+
+
+def test_ritter_dam_break():
+    check_ritter()
+
+
+def test_outflow_limited_to_available_water():
+    """A long step drains the lone wet cell exactly: the limiter caps the
+    outflow at arbase*h/dt (solver_wetdomain.cpp:309-332)."""
+    H = 0.1
+    d = flat_domain(n=10, h=0.0)
+    d.z[5, 5] += H
+    d.h[5, 5] = H
+    d.dtfl = 50.0
+    hp = d.z[5, 5] - d.zb[5, 5]               # H as the cell holds it at a 500 m datum
+    d.hydrodynamics()
+    vol_rate = 4.0 * hp / 50.0
+    out = (d.fe1[5, 5] + d.fn1[5, 5]) * 2.0
+    assert out == pytest.approx(vol_rate, rel=1e-14)
+    assert d.fe1[5, 5] > 0 and d.fn1[5, 5] > 0
+
+
+# ------------------------------------------------- Roe and turbulence ----
+def check_roe_step(lib=None):
+    """Still water with a small step in the free surface: with u = 0 the Roe
+    matrix |A| is c*I, so the mass flux through the step's face is the
+    textbook -1/2 c (z_e - z_p), c = sqrt(g h_face), h_face the mean depth
+    (solver_wetdomain.cpp:213-285), along both axes."""
+    H1, H2, n = 0.30, 0.28, 10
+    for axis in (0, 1):
+        d = flat_domain(n=n, h=H1, lib=lib, bed=0.0)
+        if axis == 0:
+            d.z[6:-1, :] = H2
+        else:
+            d.z[:, 6:-1] = H2
+        d.h[...] = d.z - d.zb
+        d.dtfl = 0.01
+        d.hydrodynamics()
+        hf = 0.5 * (H1 + H2)
+        want = 0.5 * math.sqrt(G * hf) * (H1 - H2)
+        got = d.fe1[5, 2:-2] if axis == 0 else d.fn1[2:-2, 5]
+        assert np.allclose(got, want, rtol=1e-12, atol=0), (axis, got, want)
+        # one cell further from the step nothing moves
+        away = d.fe1[3, 2:-2] if axis == 0 else d.fn1[2:-2, 3]
+        assert np.allclose(away, 0.0, atol=1e-16)
+
+
+def check_turbulent_shear(lib=None):
+    """A lake of depth H at rest in x with a y-velocity that grows linearly
+    with x (v = a + s x): the eddy-viscosity stress is nu H dv/dx, carried
+    by the y-momentum flux through east faces and, through the symmetric
+    off-diagonal term, by the x-momentum flux through north faces
+    (solver_wetdomain.cpp:177-211), with the opposite sign of the gradient."""
+    H, a, s, n, dxy = 0.2, 0.1, 0.02, 12, 2.0
+    d = flat_domain(n=n, h=H, lib=lib, bed=0.0)
+    ii = np.arange(n + 2, dtype=float)[:, None] + np.zeros((1, n + 2))
+    d.qy[...] = H * (a + s * ii)
+    d.dtfl = 0.01
+    d.hydrodynamics()
+    nu = float(np.float32(1.793e-6))
+    want = -nu * H * s / dxy
+    assert np.allclose(d.fe3[2:-2, 2:-2], want, rtol=1e-9, atol=0)
+    assert np.allclose(d.fn2[2:-2, 2:-2], want, rtol=1e-9, atol=0)
+
+
+def test_roe_flux_at_a_free_surface_step():
+    check_roe_step()
+
+
+def test_turbulent_shear_stress():
+    check_turbulent_shear()
+
+
+# -------------------------------------------------------- conservation ----
+def check_water_conservation(lib=None, steps=60):
+    rng = np.random.default_rng(7)
+    n = 16
+    d = flat_domain(n=n, h=0.0, lib=lib, ring_bed=600.0)
+    d.zb[2:-2, 2:-2] += rng.uniform(-0.2, 0.2, (n - 2, n - 2))
+    d.z[...] = d.zb
+    d.z[5:9, 5:9] += 0.3
+    d.h[...] = np.maximum(d.z - d.zb, 0.0)
+    nonweir = (d.weir == 0.0)
+    nonweir[0, :] = nonweir[-1, :] = nonweir[:, 0] = nonweir[:, -1] = False
+    v0 = d.z[nonweir].sum()
+    for _ in range(steps):
+        d.courant(1e9)
+        d.hydrodynamics()
+    v1 = d.z[nonweir].sum()
+    assert abs(v1 - v0) <= 1e-11 * abs(v0)
+
+
+def test_water_conserved_in_walled_basin():
+    check_water_conservation()
+
+
+def _ade_box(lib, pattern, fe=0.0, fn=0.0, H=0.05, n=14, D=0.01, seed=3):
+    d = flat_domain(n=n, h=H, lib=lib)
+    d.s.d_coef[0] = D
+    ring = d.weir == 1.0
+    d.ldry[1:-1, 1:-1] = 0.0
+    d.ldry[ring] = 1.0
+    d.ldry_prev[...] = d.ldry
+    d.h0[...] = d.h
+    ii, jj = np.meshgrid(np.arange(n + 2), np.arange(n + 2), indexing="ij")
+    d.conc[0][...] = pattern(ii.astype(float), jj.astype(float))
+    d.conc[0][ring] = 0.0
+    d.conc[0][0, :] = d.conc[0][-1, :] = d.conc[0][:, 0] = d.conc[0][:, -1] = 0.0
+    if callable(fe):
+        d.fe1[...] = fe(ii, jj)
+        d.fn1[...] = fn(ii, jj)
+    d.dtfl = 0.5
+    return d
+
+
+def check_ade_diffusion(lib=None):
+    """Still water, uniform depth: one sweep is the explicit 5-point diffusion
+    step with coefficient 2D (sigc = 0.5), exact on a quadratic profile."""
+    e, D, dt, dx = 0.01, 0.01, 0.5, 2.0
+    d = _ade_box(lib, lambda i, j: 1.0 + 0.1 * i + e * i * i, D=D)
+    c0 = d.conc[0].copy()
+    d.adesolver(2, 0)
+    got = d.conc[0][4:-4, 4:-4]
+    want = c0[4:-4, 4:-4] + 2 * D * dt / dx ** 2 * (2 * e)
+    assert np.allclose(got, want, rtol=0, atol=1e-13)
+
+
+def test_ade_diffusion_closed_form():
+    check_ade_diffusion()
+
+
+def test_ade_zero_stays_zero_and_uniform_stays_uniform():
+    d = _ade_box(None, lambda i, j: 0.0 * i)
+    d.adesolver(2, 0)
+    assert np.count_nonzero(d.conc[0]) == 0
+    d = _ade_box(None, lambda i, j: 0.7 + 0.0 * i)
+    d.adesolver(2, 0)
+    assert np.allclose(d.conc[0][3:-3, 3:-3], 0.7, atol=1e-15)
+
+
+def check_ade_conservation(lib=None):
+    """Advection + diffusion in a closed box of still-depth water: the sweep
+    moves solute between cells without creating or losing any.  The field is
+    monotone and the transport small, so no cell leaves the range of its
+    neighbours (the reference clamps to that range, ADEsolver_calc.cpp:284-285,
+    which is not conservative at interior extrema)."""
+    n = 14
+
+    def ramp(i, j):
+        return 1.0 + 0.02 * i + 0.015 * j
+
+    def fe(i, j):
+        f = 2e-4 * (1.0 + 0.5 * np.cos(j / 5.0)) + 0.0 * i
+        return np.where((i >= 2) & (i <= n - 2) & (j >= 2) & (j <= n - 1), f, 0.0)
+
+    def fn(i, j):
+        f = -1.5e-4 * (1.0 + 0.5 * np.sin(i / 4.0)) + 0.0 * j
+        return np.where((i >= 2) & (i <= n - 1) & (j >= 2) & (j <= n - 2), f, 0.0)
+
+    d = _ade_box(lib, ramp, fe=fe, fn=fn, n=n)
+    # the dry ring holds a high raw value: the E/N bound terms read it before
+    # pass 1 zeroes it, so no wet cell is an extreme of its neighbourhood
+    d.conc[0][d.weir == 1.0] = 100.0
+    wet = d.ldry == 0.0
+    m0 = (d.conc[0] * d.h)[wet].sum()
+    c0 = d.conc[0].copy()
+    d.adesolver(2, 0)
+    m1 = (d.conc[0] * d.h)[wet].sum()
+    assert np.abs(d.conc[0] - c0)[wet].max() > 1e-6          # something did move
+    assert abs(m1 - m0) <= 1e-12 * m0
+
+
+def test_ade_conserves_solute():
+    check_ade_conservation()
+
+
+def test_ade_skipped_on_first_solver_step():
+    d = _ade_box(None, lambda i, j: 1.0 + 0.1 * i)
+    c0 = d.conc[0].copy()
+    d.adesolver(1, 0)
+    assert np.array_equal(d.conc[0], c0)
+
+
+# ------------------------------------------------ read_geo + initiation ----
+def _dem_domain(dem, nodata=-99999.0):
+    nr, nc = dem.shape
+    d = OracleDomain(nr, nc, 2, 1e-4, 1, [0.01], True, nodata=nodata)
+    d.read_geo(dem.astype(np.float32), 1e-4)
+    d.initiation()
+    return d
+
+
+def test_read_geo_weir_ring_and_means():
+    n = 6
+    dem = np.full((n, n), 510.0)
+    dem[0, :] = dem[-1, :] = dem[:, 0] = dem[:, -1] = -99999.0
+    d = _dem_domain(dem)
+    w = d.weir[1:-1, 1:-1]
+    assert w[0, :].all() and w[-1, :].all() and w[:, 0].all() and w[:, -1].all()
+    assert not w[1:-1, 1:-1].any()
+    # interior constant bed survives the vertex averaging
+    assert np.all(d.zb[2:-2, 2:-2] == 510.0)
+    # the corner whose four vertices are all NODATA: 0/0 (initiate.cpp:85)
+    assert math.isnan(d.zb[n, n])
+    # (NROWS, 1): its SE vertex is the first-column copy taken before the ghost
+    # fill, i.e. 0 (initiate.cpp:44-51): (0 + 510 + 510) / 3
+    assert d.zb[n, 1] == pytest.approx(340.0, rel=1e-15)
+    # cold start: still water at the bed, everything dry
+    assert np.array_equal(d.z[1:-1, 1:-1], d.zb[1:-1, 1:-1], equal_nan=True)
+    assert (d.ldry[1:-1, 1:-1] == 1.0).all()
+
+
+def test_initiation_vertex_to_centre_on_a_ramp():
+    """A bed rising by 1 m per row: cell centres sit half a row up."""
+    n = 8
+    irow = np.arange(n, 0, -1, dtype=float)[:, None]     # file line 0 is irow = n
+    dem = np.broadcast_to(500.0 + irow, (n, n)).copy()
+    dem[0, :] = dem[-1, :] = dem[:, 0] = dem[:, -1] = -99999.0
+    d = _dem_domain(dem)
+    for r in range(2, n - 2):
+        assert np.allclose(d.zb[r, 2:n - 2], 500.0 + r + 0.5, rtol=0, atol=1e-12)
+    # ghosts are linear extrapolations of the first two rows (initiate.cpp:193-198)
+    assert np.allclose(d.zb[0, 3:n - 2], 1.5 * d.zb[1, 3:n - 2] - 0.5 * d.zb[2, 3:n - 2])

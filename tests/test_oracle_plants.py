@@ -1,0 +1,39 @@
+"""Each pin of the oracle catches a plausible mistake.
+
+The oracle is rebuilt with one planted mistake at a time (the ORC_PLANT_*
+switches in oracle/fluxos_oracle.c: a halved or doubled coefficient, a flipped
+sign, a shifted index, a wrong table row, a dropped term) and the pin that
+covers that part of the step must fail on it.  A pin that passes on a planted
+mistake pins nothing.
+"""
+from __future__ import annotations
+
+import pytest
+
+from oracle import oracle
+import tests.test_oracle_pins as pins
+
+PLANTS = [
+    ("PRESSURE_HALF", pins.check_lake_at_rest),   # 1/2 g h^2 -> g h^2
+    ("SMOOTH", pins.check_lake_at_rest),          # 0.8/0.1 face smoothing -> 0.9
+    ("RITTER", pins.check_ritter),                # 8/27 -> 0.3
+    ("DIFF", pins.check_ade_diffusion),           # sigc 0.5 -> 1.0
+    ("CFL_V", pins.check_cfl),                    # |v| replaced by |u|
+    ("MELT_ROW", pins.check_forcing),             # "first row > t" -> ">= t"
+    ("TURB", pins.check_turbulent_shear),         # sign of dv/dx in the stress
+    ("ROE_SIGN", pins.check_roe_step),            # sign of the Roe mass dissipation
+    ("LOOP3_INDEX", pins.check_water_conservation),  # west neighbour read as east
+    ("ADE_CARRY", pins.check_ade_conservation),   # south-face solute flux carried between columns dropped
+]
+
+
+@pytest.fixture(scope="module")
+def plant_libs():
+    return {name: str(oracle.build(plant=name)) for name, _ in PLANTS}
+
+
+@pytest.mark.parametrize("plant,check", PLANTS, ids=[p for p, _ in PLANTS])
+def test_pin_fails_on_planted_mistake(plant_libs, plant, check):
+    check()                                   # the real oracle passes
+    with pytest.raises(AssertionError):
+        check(lib=plant_libs[plant])
