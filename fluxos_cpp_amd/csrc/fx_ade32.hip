@@ -1,5 +1,5 @@
 // fx_ade32.hip -- fast advection-dispersion (FX_ADE_FAST) for the FP32-mixed
-// path, one LDS-tiled kernel for all solutes of a step.
+// path: one column-sweep kernel for up to four solutes of a step.
 //
 // The reference (ADEsolver_calc.cpp:27-291) sweeps the grid sequentially, so
 // each face sees already-updated W/S concentrations.  This form evaluates the
@@ -18,19 +18,24 @@
 //   * new concentration clamped to the min/max of the four neighbours (:284-285).
 // Each face flux is computed once per face value and used by both cells, so
 // the transport itself conserves solute mass.
+//
+// Layout of the sweep: one wave per strip of 58 rows.  Lane k holds row
+// r0-3+k of every column the wave visits; a cell's stencil reaches rows
+// r-3..r+2 (the limited flux of the W neighbour needs the W neighbour's own
+// W face), so lanes 3..60 own rows r0..r0+57.  Cross-row values move by lane
+// shuffles; cross-column values are a four-column register window (S, P, N,
+// NN) that shifts by one column per step.  Nothing is staged in LDS and every
+// cell is read from HBM once per strip (plus five halo columns per block).
+// HBM per cell: d before and after the step (FP64), the step's two face
+// discharges and the flag byte (written by the hydro sweep), and the solute
+// masses read and written.
 #include "fx_params.h"
 
 #pragma clang fp contract(fast)
 
-#define ATX 64
-#define ATY 8
-#define ALR (ATX + 5)
-#define ALC (ATY + 5)
-#define APR (ATX + 2)
-#define APC (ATY + 2)
-#define AFR (ATX + 1)
-#define AFC (ATY + 1)
-#define ANT 256
+#define AOWN 58               // rows owned per strip
+#define ACH 64                // columns per block
+#define ANC 4                 // solutes per launch
 
 namespace {
 
@@ -41,192 +46,283 @@ struct a32p {
     float *mass[2][FX_MAX_CHEM];
     const float *melt_mult;
     int64_t mm;
+    int k0;                   // first solute of this launch
 };
 
-__global__ __launch_bounds__(ANT) void k32_ade(fx_params P, a32p A, fx_ctl *ctl)
+// one cell's HBM inputs in flight
+template <int NC>
+struct apref {
+    double dold, dnew;
+    float fe, fn, mult;
+    float m[NC];
+    uint32_t f;
+};
+
+// per-cell geometry of one column, shared by the solutes
+struct ageo {
+    float h, fe, fn;
+    bool w;
+};
+
+struct actx {
+    const fx_params *P;
+    const a32p *A;
+    const double *d_old, *d_new;
+    const float *m_in[ANC];
+    float *m_out[ANC];
+    double meteoi;
+    float cm[ANC], kdiff[ANC];
+    float hdry, dx, area_dt, dt, area;
+    int row, mcols;
+    uint32_t ld;
+    bool row_ok, transport;
+};
+
+template <int NC>
+__device__ inline void a_load(const actx &X, int c, apref<NC> &a)
+{
+    a.f = 0;
+    if (!X.row_ok || c < 0 || c >= X.mcols) return;      // column test is uniform
+    const uint32_t p = (uint32_t)X.row + (uint32_t)c * X.ld;
+    const a32p &A = *X.A;
+    a.f = A.flags[p];
+    a.dold = X.d_old[p];
+    a.dnew = X.d_new[p];
+    a.fe = A.ff[p];
+    a.fn = A.ff[(uint32_t)A.mm + p];
+    a.mult = X.P->melt_mult ? X.P->melt_mult[p] : 1.0f;
+#pragma unroll
+    for (int k = 0; k < NC; k++) a.m[k] = X.m_in[k][p];
+}
+
+// derived values of a freshly loaded cell: geometry, and per solute the
+// pass-1 concentration c1 (craw where the cell was dry before the step, 0 where
+// dry now) and the raw concentration cr after the melt mixing.  Without
+// transport c1 carries the mixed mass itself, which is the update.
+template <int NC>
+__device__ inline void a_derive(const actx &X, const apref<NC> &a, ageo &g, float *c1, float *cr)
+{
+    const fx_params &P = *X.P;
+    const bool inner = a.f & FXF_INNER;
+    const bool weir = a.f & FXF_WEIR;
+    const double melt = (!inner || (a.f & FXF_NODATA)) ? 0.0 : X.meteoi * (double)a.mult;
+    const double hn = fmax(0.0, a.dnew);
+    const bool w = inner && !(hn < P.hdry) && !weir;
+    const bool wp = inner && !(fmax(0.0, a.dold) < P.hdry) && !weir;
+    const float h0 = inner ? (float)fmax(0.0, a.dold + melt) : 0.f;
+    const float mlt = (inner && a.dold > 0.0) ? (float)melt : 0.f;   // solute arrives with melt only into water
+    g.h = inner ? (float)hn : 0.f;
+    g.w = w;
+    g.fe = inner ? a.fe : 0.f;
+    g.fn = inner ? a.fn : 0.f;
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+        const float mv = inner ? a.m[k] + mlt * X.cm[k] : 0.f;
+        float c = 0.f, r = 0.f;
+        if (h0 > 0.f || w) {
+            r = h0 > 0.f ? mv / h0 : 0.f;
+            if (w) c = wp ? mv / g.h : r;
+        }
+        c1[k] = X.transport ? c : mv;
+        cr[k] = r;
+    }
+}
+
+// face flux before the outflow limiter, from the cell P (value cp, depth hp)
+// towards its neighbour Q (cq, hq); cu is the value one further upstream of P,
+// cd one further downstream of Q, wu/wd their wet flags, q the face discharge
+__device__ inline float a_face(float q, float cp, float hp, float cq, float hq, bool wq, float cu, bool wu,
+                               float cd, bool wd, float kdiff, float dx, float area_dt)
+{
+    if (!wq) return 0.f;                          // dry neighbour: fem = 0, no diffusion
+    const float hne = sqrtf(fmaxf(hp * hq, 1e-4f)) * kdiff;
+    const float pd = -hne * (cq - cp);
+    float fm;
+    if (q > 0.f) fm = wu ? -.125f * cu + .75f * cp + .375f * cq : .5f * (cp + cq);
+    else fm = wd ? .375f * cp + .75f * cq - .125f * cd : .5f * (cp + cq);
+    fm = fmaxf(0.f, fm);
+    float pf = q * fm * dx + pd;
+    if (pf < 0.f) pf = fmaxf(pf, -(cq * hq) * area_dt);
+    return pf;
+}
+
+// outflow limiter of one cell against its available solute
+__device__ inline void a_limit(float cv, float &pfe, float &qfn)
+{
+    if (cv > 0.f) {
+        if (pfe > 0.f && qfn > 0.f) {
+            if (pfe + qfn > cv) {
+                const float cf = qfn / (pfe + qfn);
+                pfe = (1.f - cf) * cv;
+                qfn = cf * cv;
+            }
+        } else if (pfe > 0.f) {
+            pfe = fminf(pfe, cv - qfn);
+        } else if (qfn > 0.f) {
+            qfn = fminf(qfn, cv - pfe);
+        }
+    } else {
+        if (pfe >= 0.f && qfn < 0.f) {
+            const float cb = cv - qfn;
+            pfe = cb > 0.f ? fminf(pfe, cb) : 0.f;
+        } else if (pfe < 0.f && qfn >= 0.f) {
+            const float cb = cv - pfe;
+            qfn = cb > 0.f ? fminf(qfn, cb) : 0.f;
+        } else if (pfe >= 0.f && qfn >= 0.f) {
+            pfe = 0.f;
+            qfn = 0.f;
+        }
+    }
+}
+
+__device__ inline float up1(float v) { return __shfl_up(v, 1, 64); }     // value of row r-1
+__device__ inline float dn1(float v) { return __shfl_down(v, 1, 64); }   // value of row r+1
+__device__ inline float dn2(float v) { return __shfl_down(v, 2, 64); }   // value of row r+2
+
+// register window of the sweep: index 0 = column c-1 (S), 1 = c (P),
+// 2 = c+1 (N), 3 = c+2 (NN)
+template <int NC>
+struct awin {
+    ageo g[4];
+    float c1[4][NC], cr[4][NC];
+    float pn_s[NC], n_s[NC];          // N-face flux of column c-1: before / after limiting
+};
+
+template <int NC>
+__device__ inline void a_shift(awin<NC> &W)
+{
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        W.g[j] = W.g[j + 1];
+#pragma unroll
+        for (int k = 0; k < NC; k++) { W.c1[j][k] = W.c1[j + 1][k]; W.cr[j][k] = W.cr[j + 1][k]; }
+    }
+}
+
+// fluxes of column c (window slot 1); with upd, also the update of the own
+// rows.  Returns nothing: the limited N-face flux is carried to column c+1.
+template <int NC>
+__device__ inline void a_column(const actx &X, awin<NC> &W, int c, bool upd, bool own)
+{
+    const ageo &gP = W.g[1], &gN = W.g[2], &gNN = W.g[3], &gS = W.g[0];
+    const bool wW = __shfl_up((int)gP.w, 1, 64), wE = __shfl_down((int)gP.w, 1, 64);
+    const bool wEE = __shfl_down((int)gP.w, 2, 64);
+    const float hE = dn1(gP.h);
+    const uint32_t p = (uint32_t)X.row + (uint32_t)c * X.ld;
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+        const float cp = W.c1[1][k];
+        float pe = 0.f, pn = 0.f;
+        const float cW = up1(cp), cE = dn1(cp), cEE = dn2(cp);
+        const float crE = dn1(W.cr[1][k]);
+        if (X.transport && gP.w) {
+            pe = a_face(gP.fe, cp, gP.h, cE, hE, wE, cW, wW, cEE, wEE, X.kdiff[k], X.dx, X.area_dt);
+            pn = a_face(gP.fn, cp, gP.h, W.c1[2][k], gN.h, gN.w, W.c1[0][k], gS.w, W.c1[3][k], gNN.w,
+                        X.kdiff[k], X.dx, X.area_dt);
+        }
+        // limiting of this cell: inflow terms of its W and S faces before their
+        // own cells' limiting
+        const float pe_w = up1(pe);
+        float e = pe, n = pn;
+        if (gP.w) a_limit(cp * gP.h * X.area_dt + pe_w + W.pn_s[k], e, n);
+        else { e = 0.f; n = 0.f; }
+        const float e_w = up1(e);
+        if (upd && own) {
+            float out;
+            if (!X.transport) {
+                out = cp;                                // melt mixing only
+            } else if (!gP.w) {
+                out = 0.f;                               // dry after the update (:69-71,119-124)
+            } else {
+                const float dc = (e_w - e + W.n_s[k] - n) * X.dt / X.area;
+                float con = cp + dc / gP.h;
+                // bounds of pass 1: W and S already rescaled, E and N not yet (:51-62)
+                const float cs = W.c1[0][k], cn = W.cr[2][k];
+                con = fminf(fmaxf(fmaxf(cW, crE), fmaxf(cs, cn)), con);
+                con = fmaxf(fminf(fminf(cW, crE), fminf(cs, cn)), con);
+                out = con * gP.h;
+            }
+            X.m_out[k][p] = out;
+        }
+        W.pn_s[k] = pn;
+        W.n_s[k] = n;
+    }
+}
+
+template <int NC>
+__global__ __launch_bounds__(64) void k32_ade_sweep(fx_params P, a32p A, fx_ctl *ctl)
 {
     if (ctl->halt) return;
     const int cur = ctl->cur;
-    const double *d_old = A.d[cur], *d_new = A.d[cur ^ 1];
-    const bool transport = ctl->gate && ctl->it > 1;
-    const double meteoi = P.meteo ? ctl->meteoi : 0.0;
+    actx X;
+    X.P = &P;
+    X.A = &A;
+    X.d_old = A.d[cur];
+    X.d_new = A.d[cur ^ 1];
+    X.transport = ctl->gate && ctl->it > 1;
+    X.meteoi = P.meteo ? ctl->meteoi : 0.0;
     const float dt = (float)ctl->dtfl;
-    const int ld = P.ld, mcols = P.ncols + 2;
-    const int r0 = 1 + blockIdx.x * ATX, c0 = 1 + blockIdx.y * ATY;
-    const int tid = threadIdx.x + threadIdx.y * 64;
-    const float hdry = (float)P.hdry;
-    const float dx = (float)P.dxy, area = (float)P.arbase;
-    const float area_dt = area / dt;
-
-    __shared__ float s_c[ALR * ALC], s_cr[ALR * ALC], s_h[ALR * ALC], s_fe[ALR * ALC], s_fn[ALR * ALC];
-    __shared__ float s_h0[ALR * ALC], s_melt[ALR * ALC];
-    __shared__ uint8_t s_w[ALR * ALC], s_wp[ALR * ALC];
-    __shared__ float s_pe[APR * APC], s_pn[APR * APC];
-    __shared__ float s_e[AFR * AFC], s_n[AFR * AFC];
-
-    // geometry shared by all solutes: depth after the update, wet now (loop-4
-    // meaning), wet before (ldry_prev), the step's face discharges
-    for (int i = tid; i < ALR * ALC; i += ANT) {
-        const int lr = i % ALR, lc = i / ALR;
-        const int r = r0 - 3 + lr, c = c0 - 3 + lc;
-        float h = 0.f, fe = 0.f, fn = 0.f, h0 = 0.f, mlt = 0.f;
-        uint8_t w = 0, wp = 0;
-        if (r >= 0 && r < ld && c >= 0 && c < mcols) {
-            const size_t p = (size_t)r + (size_t)c * ld;
-            const uint8_t f = A.flags[p];
-            if (f & FXF_INNER) {
-                const double hn = fmax(0.0, d_new[p]);
-                const double dold = d_old[p];
-                const double melt = (f & FXF_NODATA) ? 0.0
-                                    : (P.melt_mult ? meteoi * (double)A.melt_mult[p] : meteoi);
-                h = (float)hn;
-                w = (!(hn < P.hdry) && !(f & FXF_WEIR)) ? 1 : 0;
-                wp = (!(fmax(0.0, dold) < P.hdry) && !(f & FXF_WEIR)) ? 1 : 0;
-                h0 = (float)fmax(0.0, dold + melt);
-                mlt = dold > 0.0 ? (float)melt : 0.f;   // solute arrives with melt only into water
-                fe = A.ff[p];
-                fn = A.ff[A.mm + p];
-            }
-        }
-        s_h[i] = h; s_fe[i] = fe; s_fn[i] = fn; s_w[i] = w; s_wp[i] = wp;
-        s_h0[i] = h0; s_melt[i] = mlt;
+    X.hdry = (float)P.hdry;
+    X.dx = (float)P.dxy;
+    X.area_dt = (float)P.arbase / dt;
+    X.dt = dt;
+    X.area = (float)P.arbase;
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+        X.m_in[k] = A.mass[cur][A.k0 + k];
+        X.m_out[k] = A.mass[cur ^ 1][A.k0 + k];
+        X.cm[k] = (float)ctl->cmelt[A.k0 + k];
+        X.kdiff[k] = 2.f * (float)P.D[A.k0 + k];     // 1/sigc/dx*dy with sigc = 0.5, dx = dy
     }
+    const int lane = threadIdx.x;
+    const int r0 = 1 + blockIdx.x * AOWN;
+    const int c0 = 1 + blockIdx.y * ACH;
+    const int c_last = min(c0 + ACH - 1, P.ncols);
+    X.row = r0 - 3 + lane;
+    X.row_ok = X.row >= 0 && X.row < P.ld;
+    X.mcols = P.ncols + 2;
+    X.ld = (uint32_t)P.ld;
+    const bool own = lane >= 3 && lane <= 3 + AOWN - 1 && X.row <= P.nrows;
 
-    for (int k = 0; k < P.nchem; k++) {
-        const float *m = A.mass[cur][k];
-        float *m_out = A.mass[cur ^ 1][k];
-        const double cm = ctl->cmelt[k];
-        const float D = (float)P.D[k];
-        const float kdiff = 2.f * D;          // 1/sigc/dx*dy with sigc = 0.5, dx = dy
-        __syncthreads();
-        // concentration after this step's melt mixing (craw, common.cpp:226-233)
-        // and after pass 1 (c1, ADEsolver_calc.cpp:64-72): c*h0/h where wet now
-        // and before, 0 where dry now, unchanged where wet now but dry before
-        for (int i = tid; i < ALR * ALC; i += ANT) {
-            const int lr = i % ALR, lc = i / ALR;
-            const int r = r0 - 3 + lr, c = c0 - 3 + lc;
-            float c1 = 0.f, cr = 0.f;
-            if (s_h0[i] > 0.f || s_w[i]) {
-                const size_t p = (size_t)r + (size_t)c * ld;
-                const float mv = m[p] + s_melt[i] * (float)cm;
-                const float h0 = s_h0[i];
-                cr = h0 > 0.f ? mv / h0 : 0.f;
-                if (s_w[i]) c1 = s_wp[i] ? mv / s_h[i] : cr;
-            }
-            s_c[i] = transport ? c1 : cr;
-            s_cr[i] = cr;
-        }
-        __syncthreads();
-        if (transport) {
-            // face fluxes before the outflow limiter (E and N face of Q)
-            for (int i = tid; i < APR * APC; i += ANT) {
-                const int pr = i % APR, pc = i / APR;
-                const int li = (pr + 1) + (pc + 1) * ALR;     // Q at rows r0-2.., cols c0-2..
-                float pe = 0.f, pn = 0.f;
-                if (s_w[li]) {
-                    const float hp = s_h[li], cp = s_c[li];
-                    // x direction
-                    if (s_w[li + 1]) {
-                        const float he = s_h[li + 1], ce = s_c[li + 1];
-                        const float hne = sqrtf(fmaxf(hp * he, 1e-4f)) * kdiff;
-                        const float pfde = -hne * (ce - cp);
-                        const float q = s_fe[li];
-                        float fem;
-                        if (q > 0.f) fem = s_w[li - 1] ? -.125f * s_c[li - 1] + .75f * cp + .375f * ce : .5f * (cp + ce);
-                        else fem = s_w[li + 2] ? .375f * cp + .75f * ce - .125f * s_c[li + 2] : .5f * (cp + ce);
-                        fem = fmaxf(0.f, fem);
-                        pe = q * fem * dx + pfde;
-                        if (pe < 0.f) pe = fmaxf(pe, -(ce * he) * area_dt);
-                    } else {
-                        pe = 0.f;      // dry east neighbour: fem = 0, no diffusion
-                    }
-                    if (s_w[li + ALR]) {
-                        const float hn = s_h[li + ALR], cn = s_c[li + ALR];
-                        const float hnn = sqrtf(fmaxf(hp * hn, 1e-4f)) * kdiff;
-                        const float qfdn = -hnn * (cn - cp);
-                        const float q = s_fn[li];
-                        float fnm;
-                        if (q > 0.f) fnm = s_w[li - ALR] ? -.125f * s_c[li - ALR] + .75f * cp + .375f * cn : .5f * (cp + cn);
-                        else fnm = s_w[li + 2 * ALR] ? .375f * cp + .75f * cn - .125f * s_c[li + 2 * ALR] : .5f * (cp + cn);
-                        fnm = fmaxf(0.f, fnm);
-                        pn = q * fnm * dx + qfdn;
-                        if (pn < 0.f) pn = fmaxf(pn, -(cn * hn) * area_dt);
-                    } else {
-                        pn = 0.f;
-                    }
-                }
-                s_pe[i] = pe;
-                s_pn[i] = pn;
-            }
-            __syncthreads();
-            // outflow limiter of Q against its available solute
-            for (int i = tid; i < AFR * AFC; i += ANT) {
-                const int fr = i % AFR, fc = i / AFR;
-                const int li = (fr + 2) + (fc + 2) * ALR;
-                const int pi = (fr + 1) + (fc + 1) * APR;
-                float pfe = s_pe[pi], qfn = s_pn[pi];
-                if (s_w[li]) {
-                    const float pfw = s_pe[pi - 1], qfs = s_pn[pi - APR];
-                    const float cvolrat = s_c[li] * s_h[li] * area_dt + pfw + qfs;
-                    if (cvolrat > 0.f) {
-                        if (pfe > 0.f && qfn > 0.f) {
-                            if (pfe + qfn > cvolrat) {
-                                const float cf = qfn / (pfe + qfn);
-                                pfe = (1.f - cf) * cvolrat;
-                                qfn = cf * cvolrat;
-                            }
-                        } else if (pfe > 0.f) {
-                            pfe = fminf(pfe, cvolrat - qfn);
-                        } else if (qfn > 0.f) {
-                            qfn = fminf(qfn, cvolrat - pfe);
-                        }
-                    } else {
-                        if (pfe >= 0.f && qfn < 0.f) {
-                            const float cb = cvolrat - qfn;
-                            pfe = cb > 0.f ? fminf(pfe, cb) : 0.f;
-                        } else if (pfe < 0.f && qfn >= 0.f) {
-                            const float cb = cvolrat - pfe;
-                            qfn = cb > 0.f ? fminf(qfn, cb) : 0.f;
-                        } else if (pfe >= 0.f && qfn >= 0.f) {
-                            pfe = 0.f;
-                            qfn = 0.f;
-                        }
-                    }
-                } else {
-                    pfe = 0.f;
-                    qfn = 0.f;
-                }
-                s_e[i] = pfe;
-                s_n[i] = qfn;
-            }
-            __syncthreads();
-        }
-        // update own cells
-        for (int j = 0; j < ATY / 4; j++) {
-            const int lr = threadIdx.x, lc = threadIdx.y + 4 * j;
-            const int r = r0 + lr, c = c0 + lc;
-            if (r > P.nrows || c > P.ncols) continue;
-            const size_t p = (size_t)r + (size_t)c * ld;
-            const int li = (lr + 3) + (lc + 3) * ALR;
-            float out;
-            if (!transport) {
-                out = m[p] + s_melt[li] * (float)cm;
-            } else if (!s_w[li]) {
-                out = 0.f;                               // dry after the update (:69-71,119-124)
-            } else {
-                const int fi = (lr + 1) + (lc + 1) * AFR;
-                const float hp = s_h[li];
-                const float dc = (s_e[fi - 1] - s_e[fi] + s_n[fi - AFR] - s_n[fi]) * dt / area;
-                float con = s_c[li] + dc / hp;
-                // bounds of pass 1: W and S already rescaled, E and N not yet (:51-62)
-                const float cw = s_c[li - 1], ce = s_cr[li + 1], cs = s_c[li - ALR], cn = s_cr[li + ALR];
-                con = fminf(fmaxf(fmaxf(cw, ce), fmaxf(cs, cn)), con);
-                con = fmaxf(fminf(fminf(cw, ce), fminf(cs, cn)), con);
-                out = con * hp;
-            }
-            m_out[p] = out;
-        }
+    awin<NC> W;
+    apref<NC> a, b;
+    // prologue: window = columns c0-3 .. c0; the N-face fluxes of column
+    // c0-2 (before limiting) and of column c0-1 (both) feed column c0
+    a_load<NC>(X, c0 - 3, a); a_derive<NC>(X, a, W.g[0], W.c1[0], W.cr[0]);
+    a_load<NC>(X, c0 - 2, a); a_derive<NC>(X, a, W.g[1], W.c1[1], W.cr[1]);
+    a_load<NC>(X, c0 - 1, a); a_derive<NC>(X, a, W.g[2], W.c1[2], W.cr[2]);
+    a_load<NC>(X, c0, a);     a_derive<NC>(X, a, W.g[3], W.c1[3], W.cr[3]);
+#pragma unroll
+    for (int k = 0; k < NC; k++) { W.pn_s[k] = 0.f; W.n_s[k] = 0.f; }
+    a_column<NC>(X, W, c0 - 2, false, own);
+    a_shift<NC>(W);
+    a_load<NC>(X, c0 + 1, a); a_derive<NC>(X, a, W.g[3], W.c1[3], W.cr[3]);
+    a_column<NC>(X, W, c0 - 1, false, own);
+    // steady state: prefetch two columns ahead (a: c+2, b: c+3)
+    a_load<NC>(X, c0 + 2, a);
+    a_load<NC>(X, c0 + 3, b);
+    int c = c0;
+    for (; c + 1 <= c_last; c += 2) {
+        a_shift<NC>(W);
+        a_derive<NC>(X, a, W.g[3], W.c1[3], W.cr[3]);
+        if (c + 4 <= c_last + 2) a_load<NC>(X, c + 4, a);
+        a_column<NC>(X, W, c, true, own);
+        a_shift<NC>(W);
+        a_derive<NC>(X, b, W.g[3], W.c1[3], W.cr[3]);
+        if (c + 5 <= c_last + 2) a_load<NC>(X, c + 5, b);
+        a_column<NC>(X, W, c + 1, true, own);
     }
+    if (c <= c_last) {
+        a_shift<NC>(W);
+        a_derive<NC>(X, a, W.g[3], W.c1[3], W.cr[3]);
+        a_column<NC>(X, W, c, true, own);
+    }
+}
+
+template <int NC>
+void launch_nc(fx_domain d, const fx_params &P, a32p A, dim3 g)
+{
+    hipLaunchKernelGGL(k32_ade_sweep<NC>, g, dim3(64), 0, d->stream, P, A, d->ctl);
 }
 
 } // namespace
@@ -243,9 +339,17 @@ int fx32_launch_ade(fx_domain d)
         for (int k = 0; k < FX_MAX_CHEM; k++) A.mass[b][k] = d->s32.mass[b][k];
     A.melt_mult = d->melt_mult;
     A.mm = d->ncell;
-    dim3 g((P.nrows + ATX - 1) / ATX, (P.ncols + ATY - 1) / ATY);
+    const dim3 g((P.nrows + AOWN - 1) / AOWN, (P.ncols + ACH - 1) / ACH);
     fx_prof_begin(d, FX_K_ADE32);
-    hipLaunchKernelGGL(k32_ade, g, dim3(64, 4), 0, d->stream, P, A, d->ctl);
+    for (int k0 = 0; k0 < P.nchem; k0 += ANC) {
+        A.k0 = k0;
+        switch (std::min(ANC, P.nchem - k0)) {
+        case 1: launch_nc<1>(d, P, A, g); break;
+        case 2: launch_nc<2>(d, P, A, g); break;
+        case 3: launch_nc<3>(d, P, A, g); break;
+        default: launch_nc<4>(d, P, A, g); break;
+        }
+    }
     fx_prof_end(d, FX_K_ADE32);
-    return fx_hip_check(hipGetLastError(), "k32_ade");
+    return fx_hip_check(hipGetLastError(), "k32_ade_sweep");
 }
