@@ -38,6 +38,20 @@ def b_alg(precision: str, nchem: int) -> int:
     return (28 + 8 * nchem) if precision == "fp32" else (56 + 16 * nchem)
 
 
+def measured_traffic(n: int, kernel: str):
+    """HBM bytes per launch of a kernel at this workload size, as measured with
+    rocprofv3 FETCH_SIZE / WRITE_SIZE counters (profiles/hbm_traffic_<n>x<n>.txt),
+    or None when no measurement for this size is committed."""
+    f = ROOT / "profiles" / f"hbm_traffic_{n}x{n}.txt"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        parts = line.split()
+        if len(parts) == 2 and not line.startswith("#") and parts[0].split("<")[0] == kernel:
+            return int(parts[1])
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -129,6 +143,47 @@ def timed(dom, steps: int):
     return info, el, kt
 
 
+def accuracy(sc, init, snap, spans: str) -> dict:
+    """Depth error of the FP32-mixed path against the FP64 path, both advanced
+    on the device from the same snapshot to equal simulated times, beside the
+    FP64 path's own sensitivity to rounding: FP64 from the snapshot with the
+    water surface of every wet cell moved by one ulp up or down (seeded).  The
+    hydrodynamics do not depend on the solutes, so these runs carry none."""
+    from fluxos_cpp_amd import capi
+    ts = [snap["clock"][0] + float(x) for x in spans.split(",")]
+    doms = {}
+    for name, prec in (("fp64", capi.FP64_PARITY), ("fp32", capi.FP32_MIXED), ("fp64_ulp", capi.FP64_PARITY)):
+        d = capi.HipDomain.from_scenario(sc, precision=prec, ade=False, init=init)
+        restore(d, dict(snap, conc=[]))
+        doms[name] = d
+    z = doms["fp64_ulp"].get("z")
+    wet = np.zeros_like(z, dtype=bool)
+    wet[1:-1, 1:-1] = snap["h"][1:-1, 1:-1] > sc.hdry
+    up = np.random.default_rng(11).random(z.shape) < 0.5
+    z = np.where(wet, np.where(up, np.nextafter(z, np.inf), np.nextafter(z, -np.inf)), z)
+    doms["fp64_ulp"].set("z", z)
+    out = []
+    for t in ts:
+        steps = {}
+        for name, d in doms.items():
+            info = d.advance(1e30, t, 10 ** 7)
+            steps[name] = info.steps
+        h64 = doms["fp64"].get("h")[1:-1, 1:-1]
+        row = {"after_s": round(t - snap["clock"][0], 3), "steps_fp64": steps["fp64"],
+               "steps_fp32": steps["fp32"]}
+        for name in ("fp32", "fp64_ulp"):
+            dh = np.abs(doms[name].get("h")[1:-1, 1:-1] - h64)
+            row[f"{name}_max_abs_h_m"] = float(dh.max())
+            row[f"{name}_rms_h_m"] = float(np.sqrt((dh ** 2).mean()))
+        w64 = int((doms["fp64"].get("ldry")[1:-1, 1:-1] == 0).sum())
+        w32 = int((doms["fp32"].get("ldry")[1:-1, 1:-1] == 0).sum())
+        row["wet_fp64"], row["wet_fp32"] = w64, w32
+        out.append(row)
+    for d in doms.values():
+        d.close()
+    return {"from_t": round(snap["clock"][0], 2), "spans_s": out}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,8 +193,9 @@ def main() -> int:
     ap.add_argument("--fp64-steps", type=int, default=10)
     ap.add_argument("--spinup-t", type=float, default=3600.0)
     ap.add_argument("--spinup-max-steps", type=int, default=20000)
-    ap.add_argument("--accuracy-span", type=float, default=120.0)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--accuracy-spans", type=str, default="1,10,60",
+                    help="simulated seconds after the snapshot at which depths are compared")
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -173,6 +229,7 @@ def main() -> int:
 
     # FP32-mixed: spin up past the wet-up, then warm-up and timed steps
     d32 = capi.HipDomain.from_scenario(sc, precision=capi.FP32_MIXED, ade=True, init=init)
+    v0 = d32.mass_balance()[0]
     t0 = time.perf_counter()
     t_sim, nsp = spin_up(d32, args.spinup_t, args.spinup_max_steps)
     log(f"[bench] fp32 spin-up to t={t_sim:.1f}s in {nsp} steps, {time.perf_counter() - t0:.1f}s wall")
@@ -185,14 +242,30 @@ def main() -> int:
     ms_step = el * 1e3 / max(1, info.steps)
     log("[bench] fp32 kernels: " + ", ".join(f"{k}={v[0] / max(1, v[1]) * 1e3:.1f}us x{v[1]}" for k, v in kt.items()))
 
-    # roofline of the dominant kernel: the fused step (B_alg = 28 B per cell-update)
-    dom_name = max(kt, key=lambda k: kt[k][0])
-    avg_ms = kt["k32_step"][0] / kt["k32_step"][1]
-    achieved = cells * b_alg("fp32", 0) / (avg_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k32_step",
-                "bytes_per_cell": b_alg("fp32", 0), "avg_us": round(avg_ms * 1e3, 2),
-                "dominant_kernel": dom_name}
+    # water ledger of the run so far (FP64): volume now against the start plus
+    # melt input and weir-face exchange
+    v, melt, weir = d32.mass_balance()
+    mass = {"rel_err": abs(v - v0 - melt - weir) / max(abs(v), 1e-300), "volume_m3": v,
+            "melt_m3": melt, "weir_m3": weir}
+
+    # roofline of the dominant kernel (algorithmic bytes: hydro 28 B per
+    # cell-update, ADE 8 B per solute per cell-update)
+    alg = {"k32_step": b_alg("fp32", 0), "k32_ade": 8 * sc.nchem}
+    prof_name = {"k32_step": "k32_sweep", "k32_ade": "k32_ade_sweep"}
+    kernels = {}
+    for name, (ms, n) in kt.items():
+        avg_us = ms / n * 1e3
+        row = {"avg_us": round(avg_us, 2), "launches": n}
+        if name in alg:
+            gbs = cells * alg[name] / (avg_us * 1e-6) / 1e9
+            row.update(bytes_per_cell=alg[name], achieved_gbs=round(gbs, 1), frac=round(gbs / HBM_PEAK_GBS, 4))
+        kernels[name] = row
+    dom_name = max(alg, key=lambda k: kt[k][0] if k in kt else 0.0)
+    dk = kernels[dom_name]
+    roofline = {"bound": "hbm", "achieved": dk["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": dk["frac"], "traffic": measured_traffic(args.n, prof_name[dom_name]),
+                "kernel": prof_name[dom_name],
+                "bytes_per_cell": dk["bytes_per_cell"], "avg_us": dk["avg_us"]}
     step_ms = sum(v[0] for v in kt.values()) / max(1, info.steps)
     step_frac = cells * b_alg("fp32", sc.nchem) / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
 
@@ -208,28 +281,13 @@ def main() -> int:
         fp64 = {"value": round(cells * info64.solver_steps / el64, 1),
                 "ms_per_step": round(el64 * 1e3 / max(1, info64.steps), 3),
                 "steps": info64.steps}
-        # equal-time accuracy: both paths from the snapshot to snapshot + span
-        restore(d64, snap)
-        d32b = capi.HipDomain.from_scenario(sc, precision=capi.FP32_MIXED, ade=True, init=init)
-        restore(d32b, snap)
-        t_to = snap["clock"][0] + args.accuracy_span
-        i64 = d64.advance(1e30, t_to, 10 ** 7)
-        i32 = d32b.advance(1e30, t_to, 10 ** 7)
-        h64, h32 = d64.get("h")[1:-1, 1:-1], d32b.get("h")[1:-1, 1:-1]
-        w64 = int((d64.get("ldry")[1:-1, 1:-1] == 0).sum())
-        w32 = int((d32b.get("ldry")[1:-1, 1:-1] == 0).sum())
-        err = {"max_abs_h_m": float(np.nanmax(np.abs(h64 - h32))), "t_from": round(snap["clock"][0], 2),
-               "t_to": round(t_to, 2), "steps_fp64": i64.steps, "steps_fp32": i32.steps,
-               "wet_fp64": w64, "wet_fp32": w32}
-        v, melt, weir = d32.mass_balance()
-        d32b.close()
-
+        d64.close()
+        d64 = None
+        err = accuracy(sc, init, snap, args.accuracy_spans)
     cpu = None
     if not args.no_cpu:
-        src = d64 if d64 is not None else d32
-        if d64 is not None:
-            restore(d64, snap)
-        rate, cel, thr = cpu_baseline(src, sc, args.cpu_steps, args.cpu_threads or None)
+        restore(d32, snap)
+        rate, cel, thr = cpu_baseline(d32, sc, args.cpu_steps, args.cpu_threads or None)
         cpu = {"value": round(rate, 1), "unit": "cell-updates/s", "cores": thr, "kind": "port",
                "sample": f"{args.cpu_steps} loop iterations of the {args.n}x{args.n} basin from the "
                          f"GPU run's state at t={snap['clock'][0]:.0f}s ({cel:.1f}s wall; hydro loops "
@@ -244,7 +302,9 @@ def main() -> int:
                                f"2 solutes, FP32-mixed with fast ADE", "cells": cells, "nchem": sc.nchem,
                    "spinup_t": round(t_sim, 1)},
         "roofline": roofline,
+        "kernels": kernels,
         "step_roofline_frac": round(step_frac, 4),
+        "mass_balance": mass,
         "fp64_parity": fp64, "fp32_vs_fp64": err,
         "cpu_baseline": cpu,
     }

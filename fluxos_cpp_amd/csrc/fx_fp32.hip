@@ -20,10 +20,11 @@
 // Face volume terms are FP32 products summed exactly in FP64, so every face
 // moves the same volume out of one cell and into the other and the FP64
 // depth conserves water.
-// The bed is held as an FP32 high part plus an FP16 low part: differences of
+// The bed is held as an FP32 high part plus an FP32 low part: differences of
 // neighbouring high parts are exact (Sterbenz) and the low parts restore the
-// FP64 bed to ~6e-8 m, so the water-surface gradient of millimetre films over
-// a 500 m datum keeps its accuracy.
+// FP64 bed to ~1e-12 m, so the water-surface gradient of millimetre films over
+// a 500 m datum keeps its accuracy and the wet/dry and Ritter thresholds of
+// the face fluxes see the bed as the FP64 path does.
 #include "fx_params.h"
 
 #pragma clang fp contract(fast)
@@ -35,7 +36,7 @@ struct s32p {
     const double *d_in[2];
     const float *qx_in[2], *qy_in[2];
     const float *zb;
-    const _Float16 *zb_lo;
+    const float *zb_lo;
     float *f6;
     float *ff;
     uint8_t *flags;
@@ -323,7 +324,7 @@ __device__ inline void load_col(const sweep_ctx &X, int c, pref &a)
     a.qx = X.qx_in[p];
     a.qy = X.qy_in[p];
     a.zbh = S.zb[p];
-    a.zbl = (float)S.zb_lo[p];
+    a.zbl = S.zb_lo[p];
     a.mult = X.P->melt_mult ? X.P->melt_mult[p] : 1.0f;
 }
 
@@ -755,13 +756,13 @@ int fx32_set_field(fx_domain d, int field, const void *host)
     if (field == FX_F_ZB) {          // from fx_set_static: FP32 bed, flags base
         const double *zb = (const double *)host;
         std::vector<float> zf(mm);
-        std::vector<_Float16> zl(mm);
+        std::vector<float> zl(mm);
         for (int64_t i = 0; i < mm; i++) {
             zf[i] = (float)zb[i];
-            zl[i] = (_Float16)(zb[i] - (double)zf[i]);
+            zl[i] = (float)(zb[i] - (double)zf[i]);
         }
         FX_HIP(hipMemcpy(t.zb, zf.data(), mm * 4, hipMemcpyHostToDevice));
-        FX_HIP(hipMemcpy(t.zb_lo, zl.data(), mm * 2, hipMemcpyHostToDevice));
+        FX_HIP(hipMemcpy(t.zb_lo, zl.data(), mm * 4, hipMemcpyHostToDevice));
         return FX_OK;
     }
     if (field == -1) {               // weir mask (uint8 0/1) -> flag bytes
