@@ -196,7 +196,11 @@ void orc_solver_dry(orc_domain *d, int irow, int icol)
     const double hp = fmax(0.0, d->z[p] - d->z[p]);
     const double he = fmax(0.0, ze - zbe);
     const double hn = fmax(0.0, zn - zbn);
+#ifdef ORC_PLANT_DRY_BED
+    const double zbpe = zbp;
+#else
     const double zbpe = .5 * (zbe + zbp);
+#endif
     const double zbpn = .5 * (zbn + zbp);
     double hme, qme, hmn, rmn, dze = ze - zp, dzn = zn - zp;
     double volrat = 0.0;
@@ -370,7 +374,11 @@ void orc_solver_wet(orc_domain *d, int irow, int icol)
     const double txyn = hnn * ((un - up) / fabs((double)dy) + .25 * (ve + ven - vw - vwn) / dx);
 
     /* convective fluxes (:205-211) */
+#ifdef ORC_PLANT_CONV
+    const double fe1c = qme, fe2c = qme * vme, fe3c = qme * vme - txye;
+#else
     const double fe1c = qme, fe2c = qme * ume, fe3c = qme * vme - txye;
+#endif
     const double fn1c = rmn, fn2c = rmn * umn - txyn, fn3c = rmn * vmn;
 
     /* Roe dissipation (:213-285) */
@@ -638,7 +646,11 @@ void orc_adesolver(orc_domain *d, int it, int ichem, double *scratch)
             double hne = sqrt(hnue) / sigc / dx * dy * D;
             pfde = -hne * (fe - fp);
             if (qxl > 0.0) {
+#ifdef ORC_PLANT_QUICK
+                if (d->ldry[IX(d, iw, iy)] == 0) fem = -.125 * fw + .875 * fp + .25 * fe;
+#else
                 if (d->ldry[IX(d, iw, iy)] == 0) fem = -.125 * fw + .75 * fp + .375 * fe;
+#endif
                 else fem = 0.5 * fp + 0.5 * fe;
             } else {
                 if (d->ldry[IX(d, iee, iy)] == 0) fem = .375 * fp + .75 * fe - .125 * fee;

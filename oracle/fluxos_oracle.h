@@ -27,6 +27,29 @@
  * Extensions (off unless set): several solutes with their own dispersion
  * coefficient (meteo column 2+k carries solute k), and a per-cell melt
  * multiplier.  With nchem=1 and melt_mult=NULL the reference is reproduced.
+ *
+ * Pinning.  The reference ships no tests or golden vectors and cannot be
+ * built here (Armadillo is absent), so each function is checked against what
+ * the physics fixes on its own (tests/test_oracle_pins.py), and each check is
+ * shown to fail on a planted mistake (tests/test_oracle_plants.py):
+ *   orc_courant        dt = cfl dxy / (max(|u|,|v|,1e-6) + sqrt(g h)), flags
+ *   orc_add_meteo      table lookup, mm/day -> m, solute mixing, dry basin
+ *   orc_add_inflow     table lookup, q dt / dxy^2 into one cell, mixing
+ *   orc_solver_dry     Ritter inflow front (8/27, 4/9), hydrostatic face thrust
+ *   orc_solver_wet     lake at rest, uniform-flow physical flux, Roe mass flux
+ *                      at a still step, eddy-viscosity shear, Ritter outflow,
+ *                      outflow limiter
+ *   orc_hydrodynamics  water conservation in a walled basin, 0.8 smoothing
+ *   orc_adesolver      diffusion and advection closed forms, conservation,
+ *                      zero / uniform fields, first-step skip
+ *   orc_read_geo, orc_initiation   hand-worked DEMs
+ * Parity unpinned (no closed form and no reference vectors): the Roe
+ * dissipation with a nonzero face velocity and its |dz| limiter branches
+ * (solver_wetdomain.cpp:216-229, 249-259); the stale-flux persistence of
+ * all-dry neighbourhoods over several steps (solver_drydomain.cpp:60-76);
+ * the ADE's outflow limiter branches and min/max clamp under strong
+ * transport (ADEsolver_calc.cpp:243-285).  These follow the reference text
+ * line by line and are covered only by the GPU-vs-oracle parity tests.
  */
 #ifndef FLUXOS_ORACLE_H
 #define FLUXOS_ORACLE_H

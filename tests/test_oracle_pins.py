@@ -127,6 +127,38 @@ def test_melt_mixing_of_concentration():
     assert d.conc[0][6, 6] == 0.0
 
 
+def check_inflow(lib=None):
+    """Point inflow (common.cpp:277-390): the discharge of the first table row
+    later than t, q*dt/dxy^2 of depth into one cell, solute mixed by volume;
+    past the end of the table nothing enters and concentrations stay."""
+    d = flat_domain(n=8, h=0.1, lib=lib, bed=0.0)
+    d.conc[0][...] = 2.0
+    tab = np.array([[0.0, 0.0, 0.0], [100.0, 0.5, 5.0], [200.0, 1.0, 7.0]], dtype=np.float32)
+    d.set_inflow(tab, 4, 5)
+    d.tim, d.dtfl = 150.0, 0.4
+    z0 = d.z.copy()
+    d.add_inflow()
+    dz = 1.0 * 0.4 / 4.0
+    assert d.z[4, 5] == pytest.approx(0.1 + dz, rel=1e-15)
+    assert d.h[4, 5] == pytest.approx(0.1 + dz, rel=1e-15)
+    assert d.conc[0][4, 5] == pytest.approx((2.0 * 0.1 + dz * 7.0) / (0.1 + dz), rel=1e-14)
+    other = np.ones_like(z0, dtype=bool)
+    other[4, 5] = False
+    assert np.array_equal(d.z[other], z0[other])
+    d.tim = 100.0                              # row 1 is not later than t: row 2 again
+    d.add_inflow()
+    assert d.z[4, 5] == pytest.approx(0.1 + 2 * dz, rel=1e-14)
+    d.tim = 250.0                              # past the table
+    c = d.conc[0][4, 5]
+    d.add_inflow()
+    assert d.z[4, 5] == pytest.approx(0.1 + 2 * dz, rel=1e-14)
+    assert d.conc[0][4, 5] == pytest.approx(c, rel=1e-15)
+
+
+def test_inflow_point_source_closed_form():
+    check_inflow()
+
+
 def test_dry_basin_only_accumulates_melt():
     """No cell ever exceeds hdry: the solver never runs, dt is the print step,
     and the surface rises by exactly the melt."""
@@ -249,6 +281,85 @@ def check_turbulent_shear(lib=None):
     want = -nu * H * s / dxy
     assert np.allclose(d.fe3[2:-2, 2:-2], want, rtol=1e-9, atol=0)
     assert np.allclose(d.fn2[2:-2, 2:-2], want, rtol=1e-9, atol=0)
+
+
+def check_dry_face_pressure(lib=None):
+    """A dry cell whose bed stands above the water surface of its wet east /
+    north neighbour: no water moves, the face only carries the hydrostatic
+    thrust 1/2 g h_f^2 of the water standing against it, h_f = z_e - the mean
+    bed of the two cells (solver_drydomain.cpp:121-126, 146-153)."""
+    d = flat_domain(n=10, h=0.0, lib=lib, bed=0.0)
+    d.zb[5, 5] = 0.2                       # dry cell P
+    d.zb[6, 5] = 0.0                       # east neighbour, wet
+    d.zb[5, 6] = 0.0                       # north neighbour, wet
+    d.z[...] = d.zb
+    d.z[6, 5] = 0.15
+    d.z[5, 6] = 0.13
+    d.h[...] = d.z - d.zb
+    d.dtfl = 0.1
+    d.hydrodynamics()
+    he, hn = 0.15 - 0.1, 0.13 - 0.1
+    assert d.fe1[5, 5] == 0.0 and d.fn1[5, 5] == 0.0
+    assert d.fe2[5, 5] == pytest.approx(0.5 * G * he * he, rel=1e-12)
+    assert d.fn3[5, 5] == pytest.approx(0.5 * G * hn * hn, rel=1e-12)
+
+
+def check_uniform_flow(lib=None):
+    """Uniform depth H moving at u0 over a flat bed: every interior east face
+    carries the physical flux (H u0, H u0^2 + 1/2 g H^2, 0), with no Roe
+    dissipation (no jumps) and no shear; likewise along y for v0."""
+    H, u0, n = 0.2, 0.3, 12
+    for axis in (0, 1):
+        d = flat_domain(n=n, h=H, lib=lib, bed=0.0)
+        (d.qx if axis == 0 else d.qy)[...] = H * u0
+        d.dtfl = 0.01
+        d.hydrodynamics()
+        sl = (slice(2, -3), slice(2, -3))
+        if axis == 0:
+            m, p, t = d.fe1[sl], d.fe2[sl], d.fe3[sl]
+        else:
+            m, p, t = d.fn1[sl], d.fn3[sl], d.fn2[sl]
+        assert np.allclose(m, H * u0, rtol=1e-13, atol=0)
+        assert np.allclose(p, H * u0 * u0 + 0.5 * G * H * H, rtol=1e-12, atol=0)
+        assert np.allclose(t, 0.0, atol=1e-15)
+
+
+def check_ade_advection(lib=None):
+    """Uniform east-going discharge q through water of depth H carrying a
+    quadratic profile c = a + b i + e i^2: the upwind-biased face value
+    -1/8 c(i-1) + 3/4 c(i) + 3/8 c(i+1) is exactly c(i+1/2) for it, so the
+    advective change of cell i is -Cr (c(i+1/2) - c(i-1/2)) = -Cr (b + 2 e i)
+    with the cell Courant number Cr = q dt / (dx H), plus the diffusion step
+    2 D dt / dx^2 * 2e (sigc = 0.5).  The sequential sweep only adds a term of
+    second order in Cr; face weights that still sum to one but are wrong leave
+    a first-order residue e Cr * O(1)."""
+    n, H, q, dt, dx, D, a, b, e = 14, 0.05, 1e-3, 0.5, 2.0, 0.01, 1.0, 0.0, 0.02
+    d = _ade_box(lib, lambda i, j: a + b * i + e * i * i + 0.0 * j, H=H, n=n, D=D,
+                 fe=lambda i, j: np.where((i >= 1) & (j >= 2) & (j <= n - 1), q, 0.0) + 0.0 * j,
+                 fn=lambda i, j: 0.0 * i)
+    d.conc[0][d.weir == 1.0] = 100.0       # no wet cell is an extreme of its neighbourhood
+    c0 = d.conc[0].copy()
+    d.adesolver(2, 0)
+    cr = q * dt / (dx * H)
+    i = np.arange(n + 2, dtype=float)[:, None] + np.zeros((1, n + 2))
+    want = -cr * (b + 2 * e * i) + 2 * D * dt / dx ** 2 * (2 * e)
+    sl = (slice(5, -5), slice(4, -4))
+    got = (d.conc[0] - c0)[sl]
+    # tolerance Cr e / 16: ~50x the sequential sweep's second-order term here,
+    # 4x below the residue Cr e / 4 of face weights off by 1/8
+    assert np.allclose(got, want[sl], rtol=0, atol=cr * e / 16)
+
+
+def test_dry_cell_face_pressure():
+    check_dry_face_pressure()
+
+
+def test_uniform_flow_physical_flux():
+    check_uniform_flow()
+
+
+def test_ade_advects_quadratic_profile():
+    check_ade_advection()
 
 
 def test_roe_flux_at_a_free_surface_step():

@@ -20,10 +20,14 @@ PLANTS = [
     ("DIFF", pins.check_ade_diffusion),           # sigc 0.5 -> 1.0
     ("CFL_V", pins.check_cfl),                    # |v| replaced by |u|
     ("MELT_ROW", pins.check_forcing),             # "first row > t" -> ">= t"
+    ("MELT_ROW", pins.check_inflow),              # the same lookup serves the inflow table
     ("TURB", pins.check_turbulent_shear),         # sign of dv/dx in the stress
     ("ROE_SIGN", pins.check_roe_step),            # sign of the Roe mass dissipation
     ("LOOP3_INDEX", pins.check_water_conservation),  # west neighbour read as east
     ("ADE_CARRY", pins.check_ade_conservation),   # south-face solute flux carried between columns dropped
+    ("QUICK", pins.check_ade_advection),          # upwind-biased face weights 3/4, 3/8 -> 7/8, 1/4
+    ("DRY_BED", pins.check_dry_face_pressure),    # face bed = the dry cell's own bed, not the mean
+    ("CONV", pins.check_uniform_flow),            # x-momentum carried with v instead of u
 ]
 
 
