@@ -39,7 +39,9 @@ EXPORTS = ("fx_domain_create", "fx_domain_destroy", "fx_domain_bytes", "fx_set_s
            "fx_set_inflow", "fx_courant_condition", "fx_add_meteo", "fx_add_inflow",
            "fx_hydrodynamics_calc", "fx_adesolver_calc", "fx_advance", "fx_mass_balance",
            "fx_synchronize", "fx_read_geo", "fx_initiation", "fx_last_error", "fx_version",
-           "fx_profile", "fx_profile_read")
+           "fx_profile", "fx_profile_read", "fx_batch_create", "fx_batch_destroy", "fx_batch_admit",
+           "fx_batch_resume", "fx_batch_evict", "fx_batch_run", "fx_batch_poll", "fx_batch_info")
+EV_PRINT, EV_DONE, EV_ERROR = 1, 2, 3
 K_CFL, K_MELT, K_FLUX, K_UPDATE, K_ADE_PASS1, K_ADE_SWEEP, K_STEP32, K_ADE32, K_CFL32 = range(9)
 KERNEL_NAMES = {K_CFL: "k64_cfl", K_MELT: "k64_melt", K_FLUX: "k64_flux", K_UPDATE: "k64_update",
                 K_ADE_PASS1: "k_ade_pass1", K_ADE_SWEEP: "k_ade_sweep", K_STEP32: "k32_step",
@@ -58,6 +60,11 @@ class DomainDesc(C.Structure):
                 ("gacc", C.c_float), ("cfl", C.c_float), ("cvdef", C.c_float), ("nuem", C.c_float),
                 ("d_coef", C.c_double * MAX_CHEM), ("ade_solver", C.c_int32), ("precision", C.c_int32),
                 ("ade_mode", C.c_int32), ("device", C.c_int32)]
+
+
+class BatchEvent(C.Structure):
+    _fields_ = [("slot", C.c_int32), ("kind", C.c_int32), ("tim", C.c_double), ("it", C.c_int32),
+                ("steps", C.c_int32)]
 
 
 class StepInfo(C.Structure):
@@ -105,6 +112,14 @@ def load(path: Path | None = None) -> C.CDLL:
         "fx_initiation": ([C.c_int32, C.c_int32, C.c_int32, _D, _D, _D, _F, _D], C.c_int),
         "fx_profile": ([vp, C.c_int32], C.c_int),
         "fx_profile_read": ([vp, C.c_int32, _D, C.POINTER(C.c_int64)], C.c_int),
+        "fx_batch_create": ([C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(vp)], C.c_int),
+        "fx_batch_destroy": ([vp], C.c_int),
+        "fx_batch_admit": ([vp, vp, C.c_double, C.c_double, C.POINTER(C.c_int32)], C.c_int),
+        "fx_batch_resume": ([vp, C.c_int32, C.c_double], C.c_int),
+        "fx_batch_evict": ([vp, C.c_int32], C.c_int),
+        "fx_batch_run": ([vp, C.c_int32], C.c_int),
+        "fx_batch_poll": ([vp, C.POINTER(BatchEvent), C.c_int32, C.POINTER(C.c_int32)], C.c_int),
+        "fx_batch_info": ([vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
         "fx_last_error": ([], C.c_char_p),
         "fx_version": ([], C.c_char_p),
     }
@@ -358,3 +373,55 @@ def run_loop(dom: HipDomain, t_end: float, print_step: int, timstart: int = 0, o
         if info.steps == 0:
             break
     return total, solver
+
+
+class Batch:
+    """Many FP32-mixed domains advanced together (fx_batch_*).  Members keep
+    their own clock and CFL step; ``run`` enqueues batched steps and ``poll``
+    returns (slot, kind, tim, it, steps) for members that halted."""
+
+    def __init__(self, nchem: int = 1, ade: bool = False, max_members: int = 256, device: int = 0):
+        lib = load()
+        h = C.c_void_p()
+        _check(lib.fx_batch_create(device, nchem, 1 if ade else 0, max_members, C.byref(h)))
+        self.h = h
+        self.max_members = max_members
+        self._ev = (BatchEvent * max_members)()
+        self.members: dict[int, HipDomain] = {}
+
+    def close(self) -> None:
+        if self.h:
+            load().fx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def admit(self, dom: "HipDomain", t_end: float, print_next: float) -> int:
+        slot = C.c_int32()
+        _check(load().fx_batch_admit(self.h, dom.h, t_end, print_next, C.byref(slot)))
+        self.members[slot.value] = dom
+        return slot.value
+
+    def resume(self, slot: int, print_next: float) -> None:
+        _check(load().fx_batch_resume(self.h, slot, print_next))
+
+    def evict(self, slot: int) -> "HipDomain":
+        _check(load().fx_batch_evict(self.h, slot))
+        return self.members.pop(slot)
+
+    def run(self, nsteps: int) -> None:
+        _check(load().fx_batch_run(self.h, nsteps))
+
+    def poll(self) -> list[tuple[int, int, float, int, int]]:
+        n = C.c_int32()
+        _check(load().fx_batch_poll(self.h, self._ev, self.max_members, C.byref(n)))
+        return [(e.slot, e.kind, e.tim, e.it, e.steps) for e in self._ev[:n.value]]
+
+    def info(self) -> tuple[int, int, int]:
+        n, c, u = C.c_int32(), C.c_int64(), C.c_int64()
+        _check(load().fx_batch_info(self.h, C.byref(n), C.byref(c), C.byref(u)))
+        return n.value, c.value, u.value

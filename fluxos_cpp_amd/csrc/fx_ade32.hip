@@ -29,25 +29,15 @@
 // HBM per cell: d before and after the step (FP64), the step's two face
 // discharges and the flag byte (written by the hydro sweep), and the solute
 // masses read and written.
-#include "fx_params.h"
+#include "fx_kern32.h"
 
 #pragma clang fp contract(fast)
 
-#define AOWN 58               // rows owned per strip
-#define ACH 64                // columns per block
+#define AOWN FX32_AOWN        // rows owned per strip
+#define ACH FX32_ACH          // columns per block
 #define ANC 4                 // solutes per launch
 
 namespace {
-
-struct a32p {
-    const double *d[2];
-    const float *ff;
-    const uint8_t *flags;
-    float *mass[2][FX_MAX_CHEM];
-    const float *melt_mult;
-    int64_t mm;
-    int k0;                   // first solute of this launch
-};
 
 // one cell's HBM inputs in flight
 template <int NC>
@@ -249,8 +239,9 @@ __device__ inline void a_column(const actx &X, awin<NC> &W, int c, bool upd, boo
     }
 }
 
+// one block of the sweep: strip bx (rows), column block by
 template <int NC>
-__global__ __launch_bounds__(64) void k32_ade_sweep(fx_params P, a32p A, fx_ctl *ctl)
+__device__ inline void ade_tile(const fx_params &P, const a32p &A, fx_ctl *ctl, int bx, int by)
 {
     if (ctl->halt) return;
     const int cur = ctl->cur;
@@ -275,8 +266,8 @@ __global__ __launch_bounds__(64) void k32_ade_sweep(fx_params P, a32p A, fx_ctl 
         X.kdiff[k] = 2.f * (float)P.D[A.k0 + k];     // 1/sigc/dx*dy with sigc = 0.5, dx = dy
     }
     const int lane = threadIdx.x;
-    const int r0 = 1 + blockIdx.x * AOWN;
-    const int c0 = 1 + blockIdx.y * ACH;
+    const int r0 = 1 + bx * AOWN;
+    const int c0 = 1 + by * ACH;
     const int c_last = min(c0 + ACH - 1, P.ncols);
     X.row = r0 - 3 + lane;
     X.row_ok = X.row >= 0 && X.row < P.ld;
@@ -320,16 +311,40 @@ __global__ __launch_bounds__(64) void k32_ade_sweep(fx_params P, a32p A, fx_ctl 
 }
 
 template <int NC>
+__global__ __launch_bounds__(64) void k32_ade_sweep(fx_params P, a32p A, fx_ctl *ctl)
+{
+    ade_tile<NC>(P, A, ctl, blockIdx.x, blockIdx.y);
+}
+
+// batched: block b runs block FX_BMAP_BLK(amap[b]) of member FX_BMAP_SLOT(amap[b]);
+// every member of a batch carries the same number of solutes
+template <int NC>
+__global__ __launch_bounds__(64) void k32b_ade_sweep(const fx_bslot *slots, const uint32_t *amap, int k0)
+{
+    const uint32_t m = amap[blockIdx.x];
+    const fx_bslot &B = slots[FX_BMAP_SLOT(m)];
+    if (!B.ade_on) return;
+    const int blk = (int)FX_BMAP_BLK(m);
+    a32p A = B.A;
+    A.k0 = k0;
+    ade_tile<NC>(B.P, A, B.ctl, blk % B.agx, blk / B.agx);
+}
+
+template <int NC>
 void launch_nc(fx_domain d, const fx_params &P, a32p A, dim3 g)
 {
     hipLaunchKernelGGL(k32_ade_sweep<NC>, g, dim3(64), 0, d->stream, P, A, d->ctl);
 }
 
+inline dim3 ade_grid(const fx_params &P)
+{
+    return dim3((P.nrows + AOWN - 1) / AOWN, (P.ncols + ACH - 1) / ACH);
+}
+
 } // namespace
 
-int fx32_launch_ade(fx_domain d)
+a32p make_a32p(fx_domain d)
 {
-    fx_params P = fx_make_params(d);
     a32p A;
     A.d[0] = d->s32.d[0];
     A.d[1] = d->s32.d[1];
@@ -339,7 +354,33 @@ int fx32_launch_ade(fx_domain d)
         for (int k = 0; k < FX_MAX_CHEM; k++) A.mass[b][k] = d->s32.mass[b][k];
     A.melt_mult = d->melt_mult;
     A.mm = d->ncell;
-    const dim3 g((P.nrows + AOWN - 1) / AOWN, (P.ncols + ACH - 1) / ACH);
+    A.k0 = 0;
+    return A;
+}
+
+int fx32_ade_gx(fx_domain d)
+{
+    return (int)ade_grid(fx_make_params(d)).x;
+}
+
+int fx32b_launch_ade(hipStream_t st, const fx_bslot *slots, const uint32_t *amap, int nblocks, int nchem)
+{
+    for (int k0 = 0; k0 < nchem; k0 += ANC) {
+        switch (std::min(ANC, nchem - k0)) {
+        case 1: hipLaunchKernelGGL(k32b_ade_sweep<1>, dim3(nblocks), dim3(64), 0, st, slots, amap, k0); break;
+        case 2: hipLaunchKernelGGL(k32b_ade_sweep<2>, dim3(nblocks), dim3(64), 0, st, slots, amap, k0); break;
+        case 3: hipLaunchKernelGGL(k32b_ade_sweep<3>, dim3(nblocks), dim3(64), 0, st, slots, amap, k0); break;
+        default: hipLaunchKernelGGL(k32b_ade_sweep<4>, dim3(nblocks), dim3(64), 0, st, slots, amap, k0); break;
+        }
+    }
+    return fx_hip_check(hipGetLastError(), "k32b_ade_sweep");
+}
+
+int fx32_launch_ade(fx_domain d)
+{
+    fx_params P = fx_make_params(d);
+    a32p A = make_a32p(d);
+    const dim3 g = ade_grid(P);
     fx_prof_begin(d, FX_K_ADE32);
     for (int k0 = 0; k0 < P.nchem; k0 += ANC) {
         A.k0 = k0;

@@ -127,8 +127,8 @@ int64_t fp64_bytes(const fx_domain_desc &D)
 int64_t fp32_bytes(const fx_domain_desc &D)
 {
     const int64_t mm = (int64_t)(D.nrows + 2) * (D.ncols + 2);
-    // d x2 (f64), qx,qy x2, zb, f6 (6 planes), ff (2 planes), tw x2, flags
-    int64_t b = mm * 16 + mm * 4 * 4 + mm * 6 + mm * 4 * 6 + mm * 4 * 2 + mm * 4 * 2 + mm;
+    // d x2 (f64), qx,qy x2, zb hi+lo, f6 (6 planes), ff (2 planes), tw x2, flags
+    int64_t b = mm * 16 + mm * 4 * 4 + mm * 8 + mm * 4 * 6 + mm * 4 * 2 + mm * 4 * 2 + mm;
     if (D.ade_solver) b += mm * 8 * std::max(1, D.nchem);
     return b;
 }

@@ -25,25 +25,12 @@
 // FP64 bed to ~1e-12 m, so the water-surface gradient of millimetre films over
 // a 500 m datum keeps its accuracy and the wet/dry and Ritter thresholds of
 // the face fluxes see the bed as the FP64 path does.
-#include "fx_params.h"
+#include "fx_kern32.h"
 
 #pragma clang fp contract(fast)
 
 
 namespace {
-
-struct s32p {
-    const double *d_in[2];
-    const float *qx_in[2], *qy_in[2];
-    const float *zb;
-    const float *zb_lo;
-    float *f6;
-    float *ff;
-    uint8_t *flags;
-    float *tw_acc, *tw_t0;
-    double *partials;
-    int64_t mm;
-};
 
 // LDS flag bits
 #define LW 1u   // wet in loop 1
@@ -279,8 +266,8 @@ __device__ inline void cfl_cell(const fx_params &P, double h, float qx, float qy
 // each cell is read from HBM once per block and only three halo rows are
 // shared with the neighbouring block.
 // ---------------------------------------------------------------------------
-#define TH 61
-#define CH 32
+#define TH FX32_TH
+#define CH FX32_CH
 #define NS 64                 // rows held per column: r0-2 .. r0+61
 
 struct pref {                 // one cell's HBM values in flight
@@ -495,10 +482,12 @@ __device__ inline void column_step(const sweep_ctx &X, ring &R, int c, int c_las
     update_col<SC, SS, SN>(X, R, c, dtm, hmx, wv);
 }
 
-__global__ __launch_bounds__(64) void k32_sweep(fx_params P, s32p S, fx_ctl *ctl, int ade_on)
+// one block of the sweep: strip bx (rows), column block by; gx = strips per
+// column block (the block's partials go to slot bx + by * gx)
+__device__ inline void sweep_tile(const fx_params &P, const s32p &S, fx_ctl *ctl, int ade_on, int bx, int by,
+                                  int gx, ring &R)
 {
     if (ctl->halt) return;
-    __shared__ ring R;
     sweep_ctx X;
     const int cur = ctl->cur;
     X.P = &P;
@@ -514,8 +503,8 @@ __global__ __launch_bounds__(64) void k32_sweep(fx_params P, s32p S, fx_ctl *ctl
     X.meteoi = P.meteo ? ctl->meteoi : 0.0;
     X.s_wet_prev = ctl->s_wet_prev;
     X.k = threadIdx.x;
-    const int r0 = 1 + blockIdx.x * TH;
-    const int c0 = 1 + blockIdx.y * CH;
+    const int r0 = 1 + bx * TH;
+    const int c0 = 1 + by * CH;
     const int c_last = min(c0 + CH - 1, P.ncols);
     X.row = r0 - 2 + X.k;                             // this lane's row
     X.row_ok = X.row >= 0 && X.row < P.ld;
@@ -558,11 +547,27 @@ __global__ __launch_bounds__(64) void k32_sweep(fx_params P, s32p S, fx_ctl *ctl
     hmx = wmax(hmx);
     wv = wsum(wv);
     if (X.k == 0) {
-        const size_t bid = blockIdx.x + (size_t)blockIdx.y * gridDim.x;
+        const size_t bid = bx + (size_t)by * gx;
         S.partials[3 * bid] = dtm;
         S.partials[3 * bid + 1] = hmx;
         S.partials[3 * bid + 2] = wv;
     }
+}
+
+__global__ __launch_bounds__(64) void k32_sweep(fx_params P, s32p S, fx_ctl *ctl, int ade_on)
+{
+    __shared__ ring R;
+    sweep_tile(P, S, ctl, ade_on, blockIdx.x, blockIdx.y, gridDim.x, R);
+}
+
+// batched: block b runs block FX_BMAP_BLK(tmap[b]) of member FX_BMAP_SLOT(tmap[b])
+__global__ __launch_bounds__(64) void k32b_sweep(const fx_bslot *slots, const uint32_t *tmap)
+{
+    __shared__ ring R;
+    const uint32_t m = tmap[blockIdx.x];
+    const fx_bslot &B = slots[FX_BMAP_SLOT(m)];
+    const int blk = (int)FX_BMAP_BLK(m);
+    sweep_tile(B.P, B.S, B.ctl, B.ade_on, blk % B.gx, blk / B.gx, B.gx, R);
 }
 
 // CFL partials of the current state (start of a device-resident run), same
@@ -594,7 +599,7 @@ __global__ __launch_bounds__(64) void k32_cfl(fx_params P, s32p S, fx_ctl *ctl)
 // Clock of one loop iteration (main.cpp:271-325, 385-387): fold the CFL
 // partials left by the previous step, clip to the print time, advance tim,
 // evaluate the forcing, gate the solver, update the FP64 water ledger.
-__global__ __launch_bounds__(1024) void k32_clock(fx_params P, double *partials, int nblk, fx_ctl *ctl)
+__device__ inline void clock_body(const fx_params &P, double *partials, int nblk, fx_ctl *ctl)
 {
     if (ctl->halt) return;
     const int tid = threadIdx.x;
@@ -649,9 +654,44 @@ __global__ __launch_bounds__(1024) void k32_clock(fx_params P, double *partials,
     ctl->pending_halt = (ctl->tim >= ctl->print_next);
 }
 
+__global__ __launch_bounds__(1024) void k32_clock(fx_params P, double *partials, int nblk, fx_ctl *ctl)
+{
+    clock_body(P, partials, nblk, ctl);
+}
+
+// batched: one block per member
+__global__ __launch_bounds__(1024) void k32b_clock(const fx_bslot *slots)
+{
+    const fx_bslot &B = slots[blockIdx.x];
+    clock_body(B.P, B.S.partials, B.npart, B.ctl);
+}
+
 __global__ void k32_flip(fx_ctl *ctl)
 {
     if (!ctl->halt) ctl->cur ^= 1;
+}
+
+// batched end of step: the buffer flip and the print/end halt of every
+// member (k32_flip + k_step_end), and the compact status the host polls
+__global__ void k32b_end(const fx_bslot *slots, int nslot, fx_bstat *stat)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nslot) return;
+    fx_ctl *ctl = slots[i].ctl;
+    if (!ctl->halt) {
+        ctl->cur ^= 1;
+        ctl->halt = ctl->pending_halt;
+    }
+    fx_bstat st;
+    st.tim = ctl->tim;
+    st.dtfl = ctl->dtfl;
+    st.it = ctl->it;
+    st.steps = ctl->steps;
+    st.solver_steps = ctl->solver_steps;
+    st.halt = ctl->halt;
+    st.status = ctl->status;
+    st.pad = 0;
+    stat[i] = st;
 }
 
 // water volume of non-weir interior cells (FP64, block partials)
@@ -677,6 +717,8 @@ inline dim3 grid32(const fx_params &P)
     return dim3((P.nrows + TH - 1) / TH, (P.ncols + CH - 1) / CH);
 }
 
+} // namespace
+
 s32p make_s32p(fx_domain d)
 {
     fx_state32 &t = d->s32;
@@ -695,6 +737,33 @@ s32p make_s32p(fx_domain d)
     S.mm = d->ncell;
     return S;
 }
+
+int fx32_npart(fx_domain d)
+{
+    fx_params P = fx_make_params(d);
+    const dim3 g = grid32(P);
+    return (int)(g.x * g.y);
+}
+
+int fx32b_launch_clock(hipStream_t st, const fx_bslot *slots, int nslot)
+{
+    hipLaunchKernelGGL(k32b_clock, dim3(nslot), dim3(1024), 0, st, slots);
+    return fx_hip_check(hipGetLastError(), "k32b_clock");
+}
+
+int fx32b_launch_sweep(hipStream_t st, const fx_bslot *slots, const uint32_t *tmap, int nblocks)
+{
+    hipLaunchKernelGGL(k32b_sweep, dim3(nblocks), dim3(64), 0, st, slots, tmap);
+    return fx_hip_check(hipGetLastError(), "k32b_sweep");
+}
+
+int fx32b_launch_end(hipStream_t st, const fx_bslot *slots, int nslot, fx_bstat *stat)
+{
+    hipLaunchKernelGGL(k32b_end, dim3((nslot + 63) / 64), dim3(64), 0, st, slots, nslot, stat);
+    return fx_hip_check(hipGetLastError(), "k32b_end");
+}
+
+namespace {
 
 template <typename T>
 int dalloc32(fx_domain d, T **p, int64_t n)

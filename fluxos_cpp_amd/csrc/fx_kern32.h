@@ -1,0 +1,71 @@
+// fx_kern32.h -- kernel arguments of the FP32-mixed path, shared by the
+// single-domain launches and the batched ones.
+#pragma once
+
+#include "fx_params.h"
+
+// pointers of one domain's FP32-mixed state as the step sweep sees them
+struct s32p {
+    const double *d_in[2];
+    const float *qx_in[2], *qy_in[2];
+    const float *zb;
+    const float *zb_lo;
+    float *f6;
+    float *ff;
+    uint8_t *flags;
+    float *tw_acc, *tw_t0;
+    double *partials;
+    int64_t mm;
+};
+
+// pointers of one domain's state as the fast ADE sweep sees them
+struct a32p {
+    const double *d[2];
+    const float *ff;
+    const uint8_t *flags;
+    float *mass[2][FX_MAX_CHEM];
+    const float *melt_mult;
+    int64_t mm;
+    int k0;                   // first solute of this launch
+};
+
+// tiling of the FP32 kernels
+#define FX32_TH 61            // rows owned per step-sweep strip
+#define FX32_CH 32            // columns per step-sweep block
+#define FX32_AOWN 58          // rows owned per ADE-sweep strip
+#define FX32_ACH 64           // columns per ADE-sweep block
+
+// One member of a batch, in device memory: everything its kernels need.
+struct fx_bslot {
+    fx_params P;
+    s32p S;
+    a32p A;
+    fx_ctl *ctl;
+    int32_t gx;               // step-sweep blocks along rows
+    int32_t agx;              // ADE-sweep blocks along rows
+    int32_t npart;            // step-sweep blocks (CFL partials)
+    int32_t ade_on;
+};
+
+// Compact per-member status written after every batched step (read by the
+// host to find members that finished or reached their print time).
+struct fx_bstat {
+    double tim, dtfl;
+    int32_t it, steps, solver_steps, halt, status, pad;
+};
+
+s32p make_s32p(fx_domain d);
+a32p make_a32p(fx_domain d);
+int fx32_npart(fx_domain d);
+int fx32_ade_gx(fx_domain d);
+
+// batched launches over nslot members; tmap / amap give, per block, the
+// member (high 12 bits) and the member's own block index (low 20 bits)
+int fx32b_launch_clock(hipStream_t st, const fx_bslot *slots, int nslot);
+int fx32b_launch_sweep(hipStream_t st, const fx_bslot *slots, const uint32_t *tmap, int nblocks);
+int fx32b_launch_ade(hipStream_t st, const fx_bslot *slots, const uint32_t *amap, int nblocks, int nchem);
+int fx32b_launch_end(hipStream_t st, const fx_bslot *slots, int nslot, fx_bstat *stat);
+
+#define FX_BMAP(slot, blk) (((uint32_t)(slot) << 20) | (uint32_t)(blk))
+#define FX_BMAP_SLOT(v) ((v) >> 20)
+#define FX_BMAP_BLK(v) ((v) & 0xFFFFFu)

@@ -155,6 +155,44 @@ int fx_advance(fx_domain dom, double t_end, double print_next, int32_t max_steps
 int fx_mass_balance(fx_domain dom, double out[3]);
 int fx_synchronize(fx_domain dom);
 
+/* ---- batches of scenarios (extension; no counterpart in the reference) ------
+ * Many independent FP32-mixed domains advance together: one batched step runs
+ * one iteration of main.cpp:271-406 for every member, each with its own CFL
+ * dt, clock, forcing and print time, in one launch per kernel over all
+ * members' blocks (replayed from a captured hipGraph).  Members of a batch
+ * carry the same number of solutes and the same ADE setting.  A member halts
+ * when its clock reaches its print time or passes t_end; fx_batch_poll
+ * reports it, the host may then read its fields with fx_get_field and resume
+ * or evict it.  Between fx_batch_run and the next fx_batch_poll the members'
+ * state belongs to the batch: no other call may touch a member domain, and a
+ * member must be evicted before fx_domain_destroy.                          */
+typedef struct fx_batch_s *fx_batch;
+#define FX_EV_PRINT 1       /* member reached its print time (tim <= t_end)   */
+#define FX_EV_DONE  2       /* member's clock passed t_end (main.cpp:271)     */
+#define FX_EV_ERROR 3       /* NaN/Inf in the member's state                  */
+typedef struct fx_batch_event {
+    int32_t slot;        /* member slot returned by fx_batch_admit          */
+    int32_t kind;        /* FX_EV_*                                         */
+    double tim;          /* member's clock                                  */
+    int32_t it;          /* its solver-step counter                         */
+    int32_t steps;       /* loop iterations it ran since admission/resume   */
+} fx_batch_event;
+int fx_batch_create(int32_t device, int32_t nchem, int32_t ade_solver, int32_t max_members, fx_batch *out);
+int fx_batch_destroy(fx_batch batch);
+/* Admit a prepared domain (static fields, state and forcing set) that runs
+ * from its current clock until tim > t_end, halting at print_next.        */
+int fx_batch_admit(fx_batch batch, fx_domain dom, double t_end, double print_next, int32_t *slot);
+/* Continue a member halted at a print time, with its next print time. */
+int fx_batch_resume(fx_batch batch, int32_t slot, double print_next);
+/* Remove a member; the domain stays valid and keeps its state. */
+int fx_batch_evict(fx_batch batch, int32_t slot);
+/* Enqueue nsteps batched steps (asynchronous). */
+int fx_batch_run(fx_batch batch, int32_t nsteps);
+/* Wait for the queued steps; list members that halted since last listed. */
+int fx_batch_poll(fx_batch batch, fx_batch_event *events, int32_t max_events, int32_t *n_events);
+/* Members, their interior cells, and cell-updates done (as of the last poll). */
+int fx_batch_info(fx_batch batch, int32_t *n_members, int64_t *cells, int64_t *cell_updates);
+
 /* ---- host input layer (no GPU needed) -----------------------------------------
  * read_geo on a parsed DEM (read_functions.cpp:145-225): dem has NROWS lines
  * of NCOLS values in file order.  Outputs are padded grids.               */

@@ -1,0 +1,126 @@
+"""Batched FP32-mixed domains (fx_batch_*): each member's result is the one it
+gets alone.
+
+Alone, a domain runs with fx_advance (one kernel sequence per step for that
+domain); in a batch, every step is one launch per kernel over all members'
+blocks, replayed from a captured graph, with each member on its own CFL step
+and clock.  The arithmetic per cell is the same code, so the states must be
+identical bit for bit -- the stated tolerance for "the same alone and in the
+mix" is zero."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from fluxos_cpp_amd import capi, gen
+
+pytestmark = pytest.mark.gpu
+PRINT = 600.0
+
+
+def _scen(n, seed, rate):
+    sc = gen.scenario_c1(n, seed=seed, t_end=1500)
+    sc.meteo[1:, 1] = np.float32(rate)          # per-scenario melt rate [mm/day]
+    return sc
+
+
+def _alone(sc, t_end):
+    d = capi.HipDomain.from_scenario(sc, precision=capi.FP32_MIXED, ade=True)
+    pn = PRINT
+    while True:
+        info = d.advance(t_end, pn)
+        if info.status & 2:
+            break
+        if info.status & 1:
+            pn += PRINT
+    return d, info
+
+
+def _state(d):
+    return {k: d.get(k) for k in ("h", "qx", "qy", "z")} | {"c0": d.get_conc(0), "clock": np.array(d.clock()[:2])}
+
+
+def _drive(batch, next_print, done, limit=100000):
+    """Run the batch until every member has finished; resume at prints."""
+    for _ in range(limit):
+        if len(done) == len(batch.members):
+            return
+        batch.run(16)
+        for slot, kind, tim, it, steps in batch.poll():
+            assert kind != capi.EV_ERROR, f"member {slot} reported NaN/Inf"
+            if kind == capi.EV_DONE:
+                done.add(slot)
+            else:
+                next_print[slot] += PRINT
+                batch.resume(slot, next_print[slot])
+    raise AssertionError("batch did not finish")
+
+
+def test_batch_members_match_alone():
+    t_end = 1500.0
+    specs = [(64, 1, 20.0), (100, 3, 45.0), (130, 4, 70.0), (77, 5, 33.0)]
+    scens = [_scen(*s) for s in specs]
+    alone = [_state(_alone(sc, t_end)[0]) for sc in scens]
+    print("alone clocks:", [a["clock"].tolist() for a in alone], flush=True)
+
+    batch = capi.Batch(nchem=1, ade=True, max_members=8)
+    doms = [capi.HipDomain.from_scenario(sc, precision=capi.FP32_MIXED, ade=True) for sc in scens]
+    slots = [batch.admit(d, t_end, PRINT) for d in doms]
+    next_print = {s: PRINT for s in slots}
+    done: set[int] = set()
+    _drive(batch, next_print, done)
+    n, cells, upd = batch.info()
+    print(f"batch: {n} members, {cells} cells, {upd} cell-updates", flush=True)
+    for spec, slot, d, a in zip(specs, slots, doms, alone):
+        b = _state(d)
+        for k in a:
+            assert np.array_equal(a[k], b[k], equal_nan=True), f"{spec} field {k} differs"
+    batch.close()
+
+
+def test_batch_evict_and_late_admission():
+    """A member leaves mid-run and another joins late: the remaining members'
+    results are unchanged, and the late one matches its own alone run."""
+    t_end = 1300.0
+    scens = [_scen(90, 7, 50.0), _scen(70, 8, 30.0), _scen(110, 9, 60.0)]
+    alone = [_state(_alone(sc, t_end)[0]) for sc in scens]
+
+    batch = capi.Batch(nchem=1, ade=True, max_members=4)
+    d0 = capi.HipDomain.from_scenario(scens[0], precision=capi.FP32_MIXED, ade=True)
+    d1 = capi.HipDomain.from_scenario(scens[1], precision=capi.FP32_MIXED, ade=True)
+    s0 = batch.admit(d0, t_end, PRINT)
+    s1 = batch.admit(d1, t_end, PRINT)
+    next_print = {s0: PRINT, s1: PRINT}
+    # run member 1 to its first print, then evict it (its domain keeps its state)
+    for _ in range(10000):
+        batch.run(8)
+        ev = {e[0]: e for e in batch.poll()}
+        if s0 in ev:
+            next_print[s0] += PRINT
+            batch.resume(s0, next_print[s0])
+        if s1 in ev:
+            break
+    batch.evict(s1)
+    del next_print[s1]
+    # late admission of a third member into the freed slot
+    d2 = capi.HipDomain.from_scenario(scens[2], precision=capi.FP32_MIXED, ade=True)
+    s2 = batch.admit(d2, t_end, PRINT)
+    next_print[s2] = PRINT
+    done: set[int] = set()
+    _drive(batch, next_print, done)
+    for d, a, name in ((d0, alone[0], "first"), (d2, alone[2], "late")):
+        b = _state(d)
+        for k in a:
+            assert np.array_equal(a[k], b[k], equal_nan=True), f"{name} member field {k} differs"
+    # the evicted member continues alone from where it left the batch
+    pn = PRINT * 2
+    while True:
+        info = d1.advance(t_end, pn)
+        if info.status & 2:
+            break
+        if info.status & 1:
+            pn += PRINT
+    b = _state(d1)
+    for k in alone[1]:
+        assert np.array_equal(alone[1][k], b[k], equal_nan=True), f"evicted member field {k} differs"
+    batch.close()
