@@ -199,6 +199,13 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--ensemble", type=int, default=64,
+                    help="scenarios of the batched ensemble leg (0 = skip)")
+    ap.add_argument("--serve-seconds", type=float, default=5.0,
+                    help="arrival window of the served request stream (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate [1/s]")
+    ap.add_argument("--serve-rate-sustained", type=float, default=2.0,
+                    help="a second, lower arrival rate for the serving leg (0 = skip)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (calibrates HBM counters)")
     args = ap.parse_args()
@@ -293,6 +300,38 @@ def main() -> int:
                          f"GPU run's state at t={snap['clock'][0]:.0f}s ({cel:.1f}s wall; hydro loops "
                          f"OpenMP, CFL/melt/ADE serial as in the reference)"}
 
+    ensemble = None
+    if args.ensemble > 0:
+        from fluxos_cpp_amd import serve
+        t0 = time.perf_counter()
+        scens = [gen.scenario_c3(i) for i in range(args.ensemble)]
+        inits = [capi.prepare(s) for s in scens]
+        log(f"[bench] ensemble: {len(scens)} scenarios generated in {time.perf_counter() - t0:.1f}s")
+        ensemble = serve.run_ensemble(scens, inits)
+        ensemble["workload"] = (f"{args.ensemble} scenarios, side log-uniform 256-2048, dx=2 m, fractal DEMs, "
+                                "melt U(5,80) mm/day, 1 solute, 1 simulated hour each, one batch")
+        log(f"[bench] ensemble: {ensemble['cell_updates_per_s']:.3e} cell-updates/s over {ensemble['wall_s']:.1f}s")
+
+    serving = None
+    if args.serve_seconds > 0:
+        from fluxos_cpp_amd import serve
+        serving = {}
+        for label, rate in (("stated_rate", args.serve_rate), ("sustained_rate", args.serve_rate_sustained)):
+            if rate <= 0:
+                continue
+            reqs = serve.c4_stream(rate=rate, seconds=args.serve_seconds)
+            r = serve.serve(reqs)
+            r["arrival_rate_per_s"] = rate
+            r["arrival_window_s"] = args.serve_seconds
+            mean_cells = float(np.mean([q.cells for q in reqs]))
+            r["offered_cell_updates_per_s"] = rate * mean_cells * r["mean_steps"]
+            serving[label] = r
+            log(f"[bench] serving at {rate}/s: {r['requests']} requests, p50 {r['p50_s']:.2f}s "
+                f"p99 {r['p99_s']:.2f}s, {r['cell_updates_per_s']:.3e} cell-updates/s")
+        serving["workload"] = ("Poisson arrivals; side {512,1024,2048} p={0.6,0.3,0.1}; fractal DEMs; melt "
+                               "U(5,80) mm/day; 1 solute; 30 simulated minutes each; HBM budget 200 GB; "
+                               "continuous batching; time-to-result = arrival to final depth on the host")
+
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "cell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
@@ -307,6 +346,8 @@ def main() -> int:
         "mass_balance": mass,
         "fp64_parity": fp64, "fp32_vs_fp64": err,
         "cpu_baseline": cpu,
+        "ensemble": ensemble,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
     return 0

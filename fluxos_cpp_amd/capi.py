@@ -64,7 +64,7 @@ class DomainDesc(C.Structure):
 
 class BatchEvent(C.Structure):
     _fields_ = [("slot", C.c_int32), ("kind", C.c_int32), ("tim", C.c_double), ("it", C.c_int32),
-                ("steps", C.c_int32)]
+                ("steps", C.c_int32), ("solver_steps", C.c_int32), ("pad", C.c_int32)]
 
 
 class StepInfo(C.Structure):
@@ -416,10 +416,11 @@ class Batch:
     def run(self, nsteps: int) -> None:
         _check(load().fx_batch_run(self.h, nsteps))
 
-    def poll(self) -> list[tuple[int, int, float, int, int]]:
+    def poll(self) -> list[tuple[int, int, float, int, int, int]]:
+        """(slot, kind, tim, it, steps, solver_steps) of members that halted."""
         n = C.c_int32()
         _check(load().fx_batch_poll(self.h, self._ev, self.max_members, C.byref(n)))
-        return [(e.slot, e.kind, e.tim, e.it, e.steps) for e in self._ev[:n.value]]
+        return [(e.slot, e.kind, e.tim, e.it, e.steps, e.solver_steps) for e in self._ev[:n.value]]
 
     def info(self) -> tuple[int, int, int]:
         n, c, u = C.c_int32(), C.c_int64(), C.c_int64()

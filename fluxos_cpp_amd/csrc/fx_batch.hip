@@ -327,6 +327,8 @@ int fx_batch_poll(fx_batch b, fx_batch_event *ev, int32_t max_events, int32_t *n
         e.tim = st.tim;
         e.it = st.it;
         e.steps = st.steps;
+        e.solver_steps = st.solver_steps;
+        e.pad = 0;
         b->reported[s] = 1;
     }
     return FX_OK;

@@ -1,0 +1,217 @@
+"""Ensembles and a request-serving loop over batches of FP32-mixed domains.
+
+Both sit on the C ABI's batches (``fx_batch_*``): every batched step advances
+each member by one iteration of the reference's loop (main.cpp:271-406) with
+its own CFL step, in one launch per kernel over all members.
+
+* ``run_ensemble`` -- a fixed set of scenarios admitted together and run to
+  their end time (the 64-scenario ensemble).
+* ``serve`` -- continuous batching of a request stream: requests are admitted
+  as they arrive while their device memory fits the HBM budget, their state
+  stays resident while they run, members that finish leave at once (their
+  slot and memory are reused by the next arrivals), and every request's
+  time-to-result (arrival -> final state copied back) is recorded.
+
+A member halts at each print time (PRINT_STEP, as the reference's loop clips
+dt there, main.cpp:321) and is resumed right away; the final print at the
+scenario's end time is its result.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import math
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import capi, gen
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float                       # seconds after the stream starts
+    scenario: gen.Scenario
+    init: capi.InitialState
+    admitted: float = math.nan
+    finished: float = math.nan
+    steps: int = 0
+    solver_steps: int = 0
+    result: np.ndarray | None = None     # depth at the end time (padded grid)
+
+    @property
+    def cells(self) -> int:
+        return self.scenario.nrows * self.scenario.ncols
+
+
+def domain_bytes(sc: gen.Scenario, ade: bool = True) -> int:
+    """Device bytes of an FP32-mixed domain for this scenario."""
+    lib = capi.load()
+    desc = capi.DomainDesc()
+    desc.nrows, desc.ncols, desc.dxy, desc.nchem = sc.nrows, sc.ncols, sc.dxy, sc.nchem
+    desc.hdry, desc.nodata = sc.hdry, gen.NODATA
+    desc.gacc, desc.cfl, desc.cvdef, desc.nuem = 9.80665, 1.0, 0.07, 1.793e-6
+    desc.ade_solver = 1 if ade else 0
+    desc.precision = capi.FP32_MIXED
+    desc.ade_mode = capi.ADE_FAST if ade else capi.ADE_OFF
+    b = lib.fx_domain_bytes(ctypes.byref(desc))
+    if b < 0:
+        raise capi.FxError(int(b), lib.fx_last_error().decode())
+    return int(b)
+
+
+def _drive(batch: capi.Batch, t_end: dict, next_print: dict, print_step: float, on_done,
+           steps_per_poll: int) -> None:
+    """One run/poll round: resume members at intermediate prints, hand the
+    finished ones to on_done(slot, tim, steps)."""
+    batch.run(steps_per_poll)
+    for slot, kind, tim, it, steps, solver_steps in batch.poll():
+        if kind == capi.EV_ERROR:
+            raise RuntimeError(f"member {slot}: NaN/Inf in the state")
+        if kind == capi.EV_DONE or tim >= t_end[slot]:
+            on_done(slot, tim, steps, solver_steps)
+        else:
+            next_print[slot] = next_print[slot] + print_step
+            batch.resume(slot, next_print[slot])
+
+
+def run_ensemble(scens: list[gen.Scenario], inits: list[capi.InitialState], ade: bool = True,
+                 steps_per_poll: int = 16) -> dict:
+    """Admit every scenario into one batch and run all to their end time.
+    The timed region starts with every input resident on the device."""
+    nchem = max(sc.nchem for sc in scens)
+    batch = capi.Batch(nchem=nchem, ade=ade, max_members=len(scens))
+    doms, t_end, next_print = {}, {}, {}
+    for sc, st in zip(scens, inits):
+        d = capi.HipDomain.from_scenario(sc, precision=capi.FP32_MIXED, ade=ade, init=st)
+        s = batch.admit(d, float(sc.t_end), float(sc.print_step))
+        doms[s], t_end[s], next_print[s] = d, float(sc.t_end), float(sc.print_step)
+    done: dict[int, int] = {}
+    batch.poll()
+    t0 = time.perf_counter()
+    while len(done) < len(doms):
+        _drive(batch, t_end, next_print, float(scens[0].print_step),
+               lambda s, tim, steps, solver: done.__setitem__(s, steps), steps_per_poll)
+    el = time.perf_counter() - t0
+    n, cells, upd = batch.info()
+    out = {"scenarios": n, "cells": cells, "cell_updates": upd, "wall_s": el,
+           "cell_updates_per_s": upd / el, "max_steps": max(done.values()),
+           "min_steps": min(done.values())}
+    batch.close()
+    for d in doms.values():
+        d.close()
+    return out
+
+
+def poisson_arrivals(rate: float, seconds: float, seed: int) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= seconds:
+            return np.array(out)
+        out.append(t)
+
+
+def c4_stream(rate: float = 20.0, seconds: float = 300.0, seed: int = 4, pool: int = 4,
+              t_end: int = 1800) -> list[Request]:
+    """The served request stream: Poisson arrivals; per request a grid side
+    from {512, 1024, 2048} with p = {0.6, 0.3, 0.1}, a fractal DEM (one of
+    `pool` seeded DEMs per side, so inputs are prepared once) and its own
+    melt rate drawn from U(5, 80) mm/day; t_end simulated seconds each."""
+    rng = np.random.default_rng(seed + 1)
+    base = {}
+    for side in (512, 1024, 2048):
+        for k in range(pool):
+            s = 4000 + 10 * k + side
+            sc = gen.Scenario(f"c4_{side}_{k}", gen.dem_fractal(side, side, 2.0, s), 2,
+                              gen.meteo_table(np.arange(0, t_end + 1, 600), 20.0, [1.0]), 1, [0.01],
+                              t_end, s)
+            base[(side, k)] = (sc, capi.prepare(sc))
+    reqs = []
+    for i, t in enumerate(poisson_arrivals(rate, seconds, seed)):
+        side = int(rng.choice([512, 1024, 2048], p=[0.6, 0.3, 0.1]))
+        k = int(rng.integers(pool))
+        rate_i = float(rng.uniform(5.0, 80.0))
+        sc0, st = base[(side, k)]
+        sc = dataclasses.replace(sc0, name=f"c4_req{i}", seed=i,
+                                 meteo=gen.meteo_table(np.arange(0, t_end + 1, 600), rate_i, [1.0]))
+        reqs.append(Request(i, float(t), sc, st))
+    return reqs
+
+
+def serve(requests: list[Request], budget_bytes: int = 200 * 10 ** 9, max_members: int = 1024,
+          steps_per_poll: int = 8, keep_results: bool = False) -> dict:
+    """Continuous batching of a request stream (arrival times relative to the
+    start).  Returns latency percentiles and throughput."""
+    pending = deque(sorted(requests, key=lambda r: r.arrival))
+    batch = capi.Batch(nchem=1, ade=True, max_members=max_members)
+    live: dict[int, tuple[Request, capi.HipDomain, int]] = {}
+    t_end: dict[int, float] = {}
+    next_print: dict[int, float] = {}
+    used = 0
+    peak_used = 0
+    peak_live = 0
+    finished: list[Request] = []
+    t0 = time.perf_counter()
+
+    def now() -> float:
+        return time.perf_counter() - t0
+
+    def on_done(slot, tim, steps, solver_steps):
+        nonlocal used
+        req, dom, nb = live.pop(slot)
+        batch.evict(slot)
+        h = dom.get("h")                              # the result, copied back to the host
+        if keep_results:
+            req.result = h
+        dom.close()
+        used -= nb
+        req.steps = steps
+        req.solver_steps = solver_steps
+        req.finished = now()
+        finished.append(req)
+
+    while pending or live:
+        # admission: arrived requests, in order, while memory and slots allow
+        while pending and pending[0].arrival <= now():
+            req = pending[0]
+            nb = domain_bytes(req.scenario)
+            if used + nb > budget_bytes or len(live) >= max_members:
+                break
+            pending.popleft()
+            dom = capi.HipDomain.from_scenario(req.scenario, precision=capi.FP32_MIXED, ade=True,
+                                               init=req.init)
+            slot = batch.admit(dom, float(req.scenario.t_end), float(req.scenario.print_step))
+            live[slot] = (req, dom, nb)
+            t_end[slot] = float(req.scenario.t_end)
+            next_print[slot] = float(req.scenario.print_step)
+            used += nb
+            req.admitted = now()
+        peak_used = max(peak_used, used)
+        peak_live = max(peak_live, len(live))
+        if not live:
+            if pending:
+                time.sleep(max(0.0, pending[0].arrival - now()))
+            continue
+        _drive(batch, t_end, next_print, float(requests[0].scenario.print_step), on_done, steps_per_poll)
+    wall = now()
+    batch.close()
+    lat = np.array([r.finished - r.arrival for r in finished])
+    wait = np.array([r.admitted - r.arrival for r in finished])
+    upd = sum(r.cells * r.solver_steps for r in finished)
+    by_side = {}
+    for side in sorted({r.scenario.nrows for r in finished}):
+        ls = np.array([r.finished - r.arrival for r in finished if r.scenario.nrows == side])
+        by_side[str(side)] = {"n": int(ls.size), "p50_s": float(np.percentile(ls, 50)),
+                              "p99_s": float(np.percentile(ls, 99))}
+    return {"requests": len(finished), "wall_s": wall,
+            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "p99_over_p50": float(np.percentile(lat, 99) / np.percentile(lat, 50)),
+            "mean_admission_wait_s": float(wait.mean()), "cell_updates": int(upd),
+            "cell_updates_per_s": upd / wall, "peak_members": peak_live,
+            "peak_device_bytes": int(peak_used), "by_side": by_side,
+            "mean_steps": float(np.mean([r.steps for r in finished]))}

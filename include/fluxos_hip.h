@@ -175,7 +175,9 @@ typedef struct fx_batch_event {
     int32_t kind;        /* FX_EV_*                                         */
     double tim;          /* member's clock                                  */
     int32_t it;          /* its solver-step counter                         */
-    int32_t steps;       /* loop iterations it ran since admission/resume   */
+    int32_t steps;       /* loop iterations it ran since admission          */
+    int32_t solver_steps;/* of which ran the solver (hpall != 0)            */
+    int32_t pad;
 } fx_batch_event;
 int fx_batch_create(int32_t device, int32_t nchem, int32_t ade_solver, int32_t max_members, fx_batch *out);
 int fx_batch_destroy(fx_batch batch);

@@ -46,7 +46,7 @@ def _drive(batch, next_print, done, limit=100000):
         if len(done) == len(batch.members):
             return
         batch.run(16)
-        for slot, kind, tim, it, steps in batch.poll():
+        for slot, kind, tim, it, steps, solver in batch.poll():
             assert kind != capi.EV_ERROR, f"member {slot} reported NaN/Inf"
             if kind == capi.EV_DONE:
                 done.add(slot)
