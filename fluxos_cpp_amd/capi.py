@@ -40,7 +40,8 @@ EXPORTS = ("fx_domain_create", "fx_domain_destroy", "fx_domain_bytes", "fx_set_s
            "fx_hydrodynamics_calc", "fx_adesolver_calc", "fx_advance", "fx_mass_balance",
            "fx_synchronize", "fx_read_geo", "fx_initiation", "fx_last_error", "fx_version",
            "fx_profile", "fx_profile_read", "fx_batch_create", "fx_batch_destroy", "fx_batch_admit",
-           "fx_batch_resume", "fx_batch_evict", "fx_batch_run", "fx_batch_poll", "fx_batch_info")
+           "fx_batch_resume", "fx_batch_evict", "fx_batch_run", "fx_batch_poll", "fx_batch_info",
+           "fx_read_dem", "fx_read_forcing", "fx_write_results")
 EV_PRINT, EV_DONE, EV_ERROR = 1, 2, 3
 K_CFL, K_MELT, K_FLUX, K_UPDATE, K_ADE_PASS1, K_ADE_SWEEP, K_STEP32, K_ADE32, K_CFL32 = range(9)
 KERNEL_NAMES = {K_CFL: "k64_cfl", K_MELT: "k64_melt", K_FLUX: "k64_flux", K_UPDATE: "k64_update",
@@ -65,6 +66,18 @@ class DomainDesc(C.Structure):
 class BatchEvent(C.Structure):
     _fields_ = [("slot", C.c_int32), ("kind", C.c_int32), ("tim", C.c_double), ("it", C.c_int32),
                 ("steps", C.c_int32), ("solver_steps", C.c_int32), ("pad", C.c_int32)]
+
+
+class DemHeader(C.Structure):
+    _fields_ = [("nrows", C.c_int32), ("ncols", C.c_int32), ("cellsize", C.c_int32), ("pad", C.c_int32),
+                ("xllcorner", C.c_double), ("yllcorner", C.c_double), ("nodata", C.c_double)]
+
+
+class ResultsGrids(C.Structure):
+    _fields_ = [("nrows", C.c_int32), ("ncols", C.c_int32), ("dxy", C.c_int32), ("pad", C.c_int32),
+                ("xllcorner", C.c_double), ("yllcorner", C.c_double), ("h_min_print", C.c_double)] + \
+               [(k, C.POINTER(C.c_double)) for k in ("zb", "z", "h", "qx", "qy", "us", "conc0", "soil_mass",
+                                                      "fe1", "fn1", "twet")]
 
 
 class StepInfo(C.Structure):
@@ -120,6 +133,10 @@ def load(path: Path | None = None) -> C.CDLL:
         "fx_batch_run": ([vp, C.c_int32], C.c_int),
         "fx_batch_poll": ([vp, C.POINTER(BatchEvent), C.c_int32, C.POINTER(C.c_int32)], C.c_int),
         "fx_batch_info": ([vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
+        "fx_read_dem": ([C.c_char_p, C.POINTER(DemHeader), _F], C.c_int),
+        "fx_read_forcing": ([C.c_char_p, C.c_int32, C.c_int32, _F, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
+                            C.c_int),
+        "fx_write_results": ([C.c_char_p, C.POINTER(ResultsGrids)], C.c_int),
         "fx_last_error": ([], C.c_char_p),
         "fx_version": ([], C.c_char_p),
     }
@@ -183,6 +200,38 @@ class InitialState:
     ldry: np.ndarray
     conc0: np.ndarray
     melt_mult: np.ndarray | None
+
+
+def read_dem_file(path) -> tuple[DemHeader, np.ndarray]:
+    """ESRI-ASCII DEM: header and values in file order (float32)."""
+    lib = load()
+    h = DemHeader()
+    _check(lib.fx_read_dem(str(path).encode(), C.byref(h), None))
+    dem = np.zeros((h.nrows, h.ncols), dtype=np.float32)
+    _check(lib.fx_read_dem(str(path).encode(), C.byref(h), _fp(dem)))
+    return h, dem
+
+
+def read_forcing_file(path, ncols: int | None = None) -> np.ndarray:
+    """Rows of a forcing CSV after its header line (float32)."""
+    lib = load()
+    n, w = C.c_int32(), C.c_int32()
+    _check(lib.fx_read_forcing(str(path).encode(), 0, 0, None, C.byref(n), C.byref(w)))
+    cols = ncols or w.value
+    tab = np.zeros((n.value, cols), dtype=np.float32)
+    _check(lib.fx_read_forcing(str(path).encode(), n.value, cols, _fp(tab), C.byref(n), C.byref(w)))
+    return tab
+
+
+def write_results_file(path, dxy: int, xll: float, yll: float, h_min_print: float, zb, z, h, qx=None,
+                       qy=None, us=None, conc0=None, soil_mass=None, fe1=None, fn1=None, twet=None) -> None:
+    """Results/<t>.txt from padded Fortran-ordered float64 grids."""
+    keep = [None if a is None else np.asfortranarray(a, dtype=np.float64)
+            for a in (zb, z, h, qx, qy, us, conc0, soil_mass, fe1, fn1, twet)]
+    ptr = [None if a is None else _dp(a) for a in keep]
+    nr, nc = zb.shape[0] - 2, zb.shape[1] - 2
+    g = ResultsGrids(nr, nc, int(dxy), 0, float(xll), float(yll), float(h_min_print), *ptr)
+    _check(load().fx_write_results(str(path).encode(), C.byref(g)))
 
 
 def prepare(sc, nodata: float = -99999.0, ade: bool = True) -> InitialState:

@@ -205,6 +205,36 @@ int fx_read_geo(int32_t nrows, int32_t ncols, double nodata, const float *dem,
 int fx_initiation(int32_t nrows, int32_t ncols, int32_t ade_solver, double *zb,
                   double *z, double *h, float *ldry, double *conc0);
 
+/* ---- files in the reference's formats (no GPU needed) ----------------------------
+ * ESRI-ASCII DEM: NROWS/NCOLS from the first two header lines in either
+ * order (get_domain_size, common.cpp:110-161), then XLLCORNER, YLLCORNER,
+ * CELLSIZE (an integer) and NODATA_VALUE (an integer) (read_functions.cpp:
+ * 129-141).  dem (NROWS*NCOLS floats, file order; NULL = header only).     */
+typedef struct fx_dem_header {
+    int32_t nrows, ncols, cellsize, pad;
+    double xllcorner, yllcorner, nodata;
+} fx_dem_header;
+int fx_read_dem(const char *path, fx_dem_header *hdr, float *dem);
+/* Forcing CSV (read_meteo / read_inflow, read_functions.cpp:237-326): the
+ * first line is a header; each later line is one row of comma-separated
+ * numbers.  Returns the row and column counts; with table != NULL also the
+ * rows, row-major in a max_rows x max_cols float buffer (missing cells 0). */
+int fx_read_forcing(const char *path, int32_t max_rows, int32_t max_cols, float *table, int32_t *nrows,
+                    int32_t *ncols);
+/* Results/<t>.txt (write_results, write_results.cpp:30-102): one line per
+ * interior cell with zb != 0 and h > H_MIN_TO_PRINT (truncated to an integer
+ * as the reference stores it), 16 columns: irow, icol, X, Y, z, z-zb, ux,
+ * uy, qx*dxy, qy*dxy, us, conc_SW[0], soil_mass, fe_1, fn_1, twetimetracer.
+ * Numbers are written as %.16e (the reference's Armadillo CSV formatting
+ * is not pinned by any fixture).  NULL optional grids print as 0.         */
+typedef struct fx_results_grids {
+    int32_t nrows, ncols, dxy, pad;
+    double xllcorner, yllcorner, h_min_print;
+    const double *zb, *z, *h;                      /* required */
+    const double *qx, *qy, *us, *conc0, *soil_mass, *fe1, *fn1, *twet;
+} fx_results_grids;
+int fx_write_results(const char *path, const fx_results_grids *grids);
+
 /* ---- kernel timing (extension; no counterpart in the reference) --------------
  * When enabled, every kernel the domain launches is bracketed by HIP events
  * recorded on the domain's own stream.  fx_profile_read returns the summed

@@ -1,0 +1,157 @@
+"""The file formats at the boundary (fx_read_dem, fx_read_forcing,
+fx_write_results) and the command-line driver's input handling, on the CPU.
+
+Formats follow read_functions.cpp:112-326, common.cpp:110-161 and
+write_results.cpp:30-102.  The reference ships no fixtures for them, so the
+checks use files written by this package's generator and hand-made edge
+cases (header order, integer CELLSIZE, the unsigned-long print threshold)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from fluxos_cpp_amd import build, capi, gen
+
+
+DemHeader, ResultsGrids = capi.DemHeader, capi.ResultsGrids
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return capi.load()
+
+
+def read_dem(lib, path):
+    h = DemHeader()
+    assert lib.fx_read_dem(str(path).encode(), C.byref(h), None) == 0, lib.fx_last_error()
+    dem = np.zeros((h.nrows, h.ncols), dtype=np.float32)
+    assert lib.fx_read_dem(str(path).encode(), C.byref(h), dem.ctypes.data_as(C.POINTER(C.c_float))) == 0
+    return h, dem
+
+
+def test_dem_round_trip(lib, tmp_path):
+    sc = gen.scenario_c1(40, seed=3)
+    gen.write_dem(tmp_path / "a.asc", sc.dem, sc.dxy, xll=425000.5, yll=5785000.0)
+    h, dem = read_dem(lib, tmp_path / "a.asc")
+    assert (h.nrows, h.ncols, h.cellsize) == (40, 40, 2)
+    assert h.nodata == gen.NODATA and h.xllcorner == 425000.5 and h.yllcorner == 5785000.0
+    assert np.array_equal(dem, sc.dem)
+
+
+def test_dem_header_as_the_reference_reads_it(lib, tmp_path):
+    """NCOLS/NROWS in either order and case, CELLSIZE and NODATA read as the
+    first integer of their line (getIntNumberFromString: '2.7' -> 2)."""
+    p = tmp_path / "b.asc"
+    vals = np.arange(12, dtype=np.float32).reshape(3, 4) + 500.25
+    with open(p, "w") as f:
+        f.write("nrows 3\nncols 4\nxllcorner 10.5\nyllcorner -2\ncellsize 2.7\nNODATA_value -9999\n")
+        for row in vals:
+            f.write("  ".join(f"{v:.9g}" for v in row) + "  \n")
+    h, dem = read_dem(lib, p)
+    assert (h.nrows, h.ncols, h.cellsize, h.nodata) == (3, 4, 2, -9999.0)
+    assert h.xllcorner == 10.5 and h.yllcorner == -2.0
+    assert np.array_equal(dem, vals)
+
+
+def test_dem_errors(lib, tmp_path):
+    h = DemHeader()
+    rc = lib.fx_read_dem(str(tmp_path / "missing.asc").encode(), C.byref(h), None)
+    assert rc == -6 and b"cannot open" in lib.fx_last_error()
+    (tmp_path / "short.asc").write_text("NCOLS 4\nNROWS 3\n")
+    assert lib.fx_read_dem(str(tmp_path / "short.asc").encode(), C.byref(h), None) == -6
+
+
+def test_forcing_csv(lib, tmp_path):
+    sc = gen.scenario_c2(64)
+    gen.write_meteo(tmp_path / "m.csv", sc.meteo)
+    n, w = C.c_int32(), C.c_int32()
+    path = str(tmp_path / "m.csv").encode()
+    assert lib.fx_read_forcing(path, 0, 0, None, C.byref(n), C.byref(w)) == 0
+    assert (n.value, w.value) == (sc.meteo.shape[0] - 1, 4)
+    tab = np.zeros((n.value, 5), dtype=np.float32)          # one spare column stays 0
+    assert lib.fx_read_forcing(path, n.value, 5, tab.ctypes.data_as(C.POINTER(C.c_float)),
+                               C.byref(n), C.byref(w)) == 0
+    assert np.array_equal(tab[:, :4], sc.meteo[1:]) and not tab[:, 4].any()
+    small = np.zeros((2, 4), dtype=np.float32)
+    assert lib.fx_read_forcing(path, 2, 4, small.ctypes.data_as(C.POINTER(C.c_float)),
+                               C.byref(n), C.byref(w)) == -1
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def test_results_writer_filter_order_and_columns(lib, tmp_path):
+    rng = np.random.default_rng(5)
+    nr, nc, dxy = 5, 4, 3
+    shape = (nr + 2, nc + 2)
+    zb = np.asfortranarray(rng.uniform(500, 501, shape))
+    h = np.asfortranarray(np.where(rng.random(shape) < 0.4, 0.0, rng.uniform(0, 2.0, shape)))
+    zb[2, 3] = 0.0                      # bed exactly 0 is never printed
+    h[2, 3] = 1.0
+    z = np.asfortranarray(zb + h + 1e-3)
+    f = {k: np.asfortranarray(rng.normal(size=shape)) for k in ("qx", "qy", "conc0", "fe1", "fn1", "twet")}
+    for hmin, thresh in ((0.005, 0.0), (1.5, 1.0)):
+        g = ResultsGrids(nr, nc, dxy, 0, 100.0, 200.0, hmin, _dp(zb), _dp(z), _dp(h), _dp(f["qx"]), _dp(f["qy"]),
+                         None, _dp(f["conc0"]), None, _dp(f["fe1"]), _dp(f["fn1"]), _dp(f["twet"]))
+        out = tmp_path / f"r{hmin}.txt"
+        assert lib.fx_write_results(str(out).encode(), C.byref(g)) == 0, lib.fx_last_error()
+        lines = out.read_text().splitlines()
+        assert lines[0].split(",")[0] == "irow [-]" and len(lines[0].split(",")) == 16
+        got = np.array([[float(x) for x in ln.split(",")] for ln in lines[1:]])
+        want = []
+        for icol in range(1, nc + 1):                   # column-major order, irow fastest
+            for irow in range(1, nr + 1):
+                if zb[irow, icol] != 0.0 and h[irow, icol] > thresh:
+                    want.append([irow, icol, icol * dxy + 100.0, irow * dxy + 200.0, z[irow, icol],
+                                 z[irow, icol] - zb[irow, icol], 0, 0, f["qx"][irow, icol] * dxy,
+                                 f["qy"][irow, icol] * dxy, 0, f["conc0"][irow, icol], 0, f["fe1"][irow, icol],
+                                 f["fn1"][irow, icol], f["twet"][irow, icol]])
+        assert np.array_equal(got, np.array(want))     # %.16e round-trips doubles exactly
+
+
+def _gpu_visible() -> bool:
+    try:
+        import torch
+        return torch.cuda.device_count() > 0
+    except Exception:
+        return False
+
+
+@pytest.mark.skipif(_gpu_visible(), reason="checks the no-GPU failure path")
+def test_cli_reads_inputs_then_fails_loudly_without_gpu(tmp_path):
+    """The driver parses the modset (with comments), reads the DEM and the
+    forcing, runs read_geo + initiation, and stops at domain creation with the
+    library's no-device error -- there is no CPU fallback."""
+    cli = build.CLI
+    if not cli.exists():
+        build.build_cli()
+    sc = gen.scenario_c1(32, seed=2, t_end=1200)
+    modset = gen.write_case(tmp_path, sc)
+    text = modset.read_text()
+    modset.write_text(text.replace('"RESTART": false,', '"RESTART": false, /* block comment */'))
+    r = subprocess.run([str(cli), str(modset)], capture_output=True, text=True, timeout=60)
+    out = r.stdout + r.stderr
+    assert r.returncode != 0
+    assert "Successful loading of master input file" in out
+    assert "Successful loading of DEM file" in out and "Successful loading of METEO file" in out
+    assert "Simulation time (days) = 1200" in out
+    assert "no HIP device" in out
+    assert (tmp_path / "fluxos_run.log").exists()
+
+
+def test_cli_rejects_unsupported_modules(tmp_path):
+    cli = build.CLI
+    if not cli.exists():
+        build.build_cli()
+    sc = gen.scenario_c1(16, seed=2, t_end=600)
+    modset = gen.write_case(tmp_path, sc)
+    text = modset.read_text().replace('"RESTART": false', '"RESTART": true')
+    modset.write_text(text)
+    r = subprocess.run([str(cli), str(modset)], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "RESTART" in r.stdout
