@@ -9,11 +9,14 @@
 //   k32b_ade     every member's ADE blocks (if the batch carries solutes);
 //   k32b_end     buffer flip, print/end halt and a compact status per member.
 // The per-block member map (tmap/amap) is rebuilt when members join or
-// leave; a run of several steps is captured once as a hipGraph and replayed.
+// leave; a run of several steps is captured once as a hipGraph and replayed
+// (environment FLUXOS_BATCH_GRAPH=0 launches every step directly instead,
+// for tools that do not follow graph launches).
 // A member whose clock reaches its print time (or passes t_end) halts: its
 // blocks return at once until the host resumes or evicts it, so members
 // finish independently while the rest keep stepping.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -43,6 +46,7 @@ struct fx_batch_s {
     int ntiles = 0, natiles = 0, nslot = 0;
     fx_ctl *idle_ctl = nullptr;        // halted control block of free slots
     bool dirty = true;
+    bool use_graph = true;             // FLUXOS_BATCH_GRAPH=0: direct launches
     hipGraphExec_t graph = nullptr;
     bool graph_ok = false;
     int64_t steps_run = 0;
@@ -158,6 +162,10 @@ int fx_batch_create(int32_t device, int32_t nchem, int32_t ade_solver, int32_t m
     b->nchem = std::max(1, (int)nchem);
     b->ade = ade_solver ? 1 : 0;
     b->maxm = max_members;
+    {
+        const char *g = std::getenv("FLUXOS_BATCH_GRAPH");
+        b->use_graph = !(g && g[0] == '0');
+    }
     b->slots_host.assign(max_members, fx_bslot{});
     b->member.assign(max_members, nullptr);
     b->reported.assign(max_members, 0);
@@ -295,7 +303,7 @@ int fx_batch_run(fx_batch b, int32_t nsteps)
     if ((rc = set_dev(b))) return rc;
     if (b->dirty && (rc = rebuild(b))) return rc;
     if (b->nslot == 0) return FX_OK;
-    if (!b->graph_ok && nsteps >= GRAPH_STEPS) (void)capture(b);
+    if (b->use_graph && !b->graph_ok && nsteps >= GRAPH_STEPS) (void)capture(b);
     int left = nsteps;
     while (b->graph_ok && left >= GRAPH_STEPS) {
         FX_HIP(hipGraphLaunch(b->graph, b->stream));
