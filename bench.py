@@ -1,18 +1,24 @@
 #!/usr/bin/env python3
 """Benchmark of the FLUXOS-OVERLAND time step on one MI355X.
 
-Workload (the single large domain the metric is quoted on): a 4096 x 4096
-basin, dx = 2 m, diamond-square fractal DEM (roughness 0.6, relief 8 m) over a
-0.3 % slope, melt ramping 0 -> 60 mm/day over 6 h, two solutes.  The run is
-spun up on the device past the time the melt wets the basin; then W warm-up
-steps and K timed steps of the hot path (CFL -> forcing -> fluxes -> update ->
-ADE per solute) run device-resident.  A step is one loop iteration of the
-reference's main.cpp:271-406 with the solver active; a cell-update is one
-interior cell advanced by one such step.
+Headline workload (the single large domain the metric is quoted on): a 4096 x
+4096 basin, dx = 2 m, diamond-square fractal DEM (roughness 0.6, relief 8 m)
+over a 0.3 % slope, melt ramping 0 -> 60 mm/day over 6 h, two solutes, run on
+the FP32-mixed path for the whole 6 simulated hours (about 10^5 loop
+iterations), device-resident between print times, after W warm-up iterations
+of a throwaway domain.  A step is one loop iteration of the reference's
+main.cpp:271-406; a cell-update is one interior cell advanced by one step
+that ran the solver (hpall != 0).
 
-Prints ONE JSON line (rank 0): metric, value (cell-updates/s), roofline of the
-dominant kernel (HIP events on the library's own stream), and the reference
-CPU path (the oracle, OpenMP) timed on this host on a bounded sample.
+Beside it, from a snapshot of the same run at t = 3600 s: the FP64 parity path
+and FP32-mixed over the same window, FP32 depth error against FP64 next to
+FP64's own rounding sensitivity, and the reference's CPU path (the oracle,
+OpenMP) on a bounded sample; then the 64-scenario batched ensemble and the
+served request stream (p50 / p99 time-to-result).
+
+Prints ONE JSON line: metric, value (cell-updates/s), roofline of the dominant
+kernel (HIP events on the library's own stream over the timed run), and the
+legs above.
 """
 from __future__ import annotations
 
@@ -187,10 +193,11 @@ def accuracy(sc, init, snap, spans: str) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=0,
+                    help="timed loop iterations (0 = the whole run, t = 0 .. t_end)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=4096)
-    ap.add_argument("--fp64-steps", type=int, default=10)
+    ap.add_argument("--fp64-steps", type=int, default=20)
     ap.add_argument("--spinup-t", type=float, default=3600.0)
     ap.add_argument("--spinup-max-steps", type=int, default=20000)
     ap.add_argument("--accuracy-spans", type=str, default="1,10,60",
@@ -234,19 +241,46 @@ def main() -> int:
     log(f"[bench] generated {args.n}^2 basin in {time.perf_counter() - t0:.1f}s")
     cells = sc.nrows * sc.ncols
 
-    # FP32-mixed: spin up past the wet-up, then warm-up and timed steps
+    # warm-up: W loop iterations of a throwaway domain (code objects, first touch)
+    dw = capi.HipDomain.from_scenario(sc, precision=capi.FP32_MIXED, ade=True, init=init)
+    dw.advance(float(sc.t_end), 1e30, args.warmup)
+    dw.synchronize()
+    dw.close()
+
+    # timed: the whole run, t = 0 .. t_end, device-resident between print times
+    # (PRINT_STEP as in the reference; the host only reads the clock there)
     d32 = capi.HipDomain.from_scenario(sc, precision=capi.FP32_MIXED, ade=True, init=init)
     v0 = d32.mass_balance()[0]
-    t0 = time.perf_counter()
-    t_sim, nsp = spin_up(d32, args.spinup_t, args.spinup_max_steps)
-    log(f"[bench] fp32 spin-up to t={t_sim:.1f}s in {nsp} steps, {time.perf_counter() - t0:.1f}s wall")
-    snap = snapshot(d32)
-    d32.advance(1e30, 1e30, args.warmup)
-    info, el, kt = timed(d32, args.steps)
-    if info.solver_steps != args.steps:
-        log(f"[bench] warning: {info.steps} steps, {info.solver_steps} with the solver")
-    value = cells * info.solver_steps / el
-    ms_step = el * 1e3 / max(1, info.steps)
+    if args.steps > 0:                  # only the first K iterations of the run
+        info, el, kt = timed(d32, args.steps)
+        steps, solver_steps, t_reached = info.steps, info.solver_steps, info.tim
+    else:
+        d32.synchronize()
+        d32.profile(True)
+        d32.synchronize()
+        steps = solver_steps = 0
+        print_next = float(sc.print_step)
+        t0 = time.perf_counter()
+        while True:
+            info = d32.advance(float(sc.t_end), print_next)
+            steps += info.steps
+            solver_steps += info.solver_steps
+            if info.status & 2 or info.steps == 0:
+                break
+            if info.status & 1:
+                print_next += sc.print_step
+        d32.synchronize()
+        el = time.perf_counter() - t0
+        t_reached = info.tim
+        kt = {}
+        for kid, name in capi.KERNEL_NAMES.items():
+            ms, n = d32.profile_read(kid)
+            if n:
+                kt[name] = (ms, n)
+        d32.profile(False)
+    value = cells * solver_steps / el
+    ms_step = el * 1e3 / max(1, steps)
+    log(f"[bench] fp32 run to t={t_reached:.1f}s: {steps} steps ({solver_steps} with the solver) in {el:.1f}s")
     log("[bench] fp32 kernels: " + ", ".join(f"{k}={v[0] / max(1, v[1]) * 1e3:.1f}us x{v[1]}" for k, v in kt.items()))
 
     # water ledger of the run so far (FP64): volume now against the start plus
@@ -273,21 +307,34 @@ def main() -> int:
                 "frac": dk["frac"], "traffic": measured_traffic(args.n, prof_name[dom_name]),
                 "kernel": prof_name[dom_name],
                 "bytes_per_cell": dk["bytes_per_cell"], "avg_us": dk["avg_us"]}
-    step_ms = sum(v[0] for v in kt.values()) / max(1, info.steps)
+    step_ms = sum(v[0] for v in kt.values()) / max(1, solver_steps)
     step_frac = cells * b_alg("fp32", sc.nchem) / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
 
-    # FP64 parity path from the same snapshot, and the depth error at equal time
+    # a snapshot of the run at t = spinup_t for the comparisons below
+    d32.close()
+    d32 = capi.HipDomain.from_scenario(sc, precision=capi.FP32_MIXED, ade=True, init=init)
+    t0 = time.perf_counter()
+    t_sim, nsp = spin_up(d32, args.spinup_t, args.spinup_max_steps)
+    log(f"[bench] snapshot: fp32 spin-up to t={t_sim:.1f}s in {nsp} steps, {time.perf_counter() - t0:.1f}s wall")
+    snap = snapshot(d32)
+
+    # FP64 parity path and FP32-mixed over the same window from the snapshot,
+    # and the depth error at equal time
     fp64 = None
     err = None
     d64 = None
     if not args.no_fp64:
+        restore(d32, snap)
+        info_w, el_w, _ = timed(d32, args.fp64_steps)
         d64 = capi.HipDomain.from_scenario(sc, precision=capi.FP64_PARITY, ade=True, init=init)
         restore(d64, snap)
         info64, el64, kt64 = timed(d64, args.fp64_steps)
         log("[bench] fp64 kernels: " + ", ".join(f"{k}={v[0] / max(1, v[1]) * 1e3:.1f}us x{v[1]}" for k, v in kt64.items()))
-        fp64 = {"value": round(cells * info64.solver_steps / el64, 1),
-                "ms_per_step": round(el64 * 1e3 / max(1, info64.steps), 3),
-                "steps": info64.steps}
+        v64 = cells * info64.solver_steps / el64
+        v32w = cells * info_w.solver_steps / el_w
+        fp64 = {"value": round(v64, 1), "ms_per_step": round(el64 * 1e3 / max(1, info64.steps), 3),
+                "steps": info64.steps, "window_from_t": round(snap["clock"][0], 1),
+                "fp32_value_same_window": round(v32w, 1), "fp32_over_fp64": round(v32w / v64, 1)}
         d64.close()
         d64 = None
         err = accuracy(sc, init, snap, args.accuracy_spans)
@@ -337,9 +384,11 @@ def main() -> int:
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 (f64 depth, dt, mass)", "data": "synthetic",
-        "config": {"workload": f"single domain {args.n}x{args.n}, dx=2 m, fractal DEM, melt 0-60 mm/day, "
-                               f"2 solutes, FP32-mixed with fast ADE", "cells": cells, "nchem": sc.nchem,
-                   "spinup_t": round(t_sim, 1)},
+        "config": {"workload": f"single domain {args.n}x{args.n}, dx=2 m, fractal DEM + 0.3% slope, melt 0-60 "
+                               f"mm/day over 6 h, 2 solutes, FP32-mixed with fast ADE, "
+                               + ("the whole 6 simulated hours" if args.steps == 0 else f"first {args.steps} iterations"),
+                   "cells": cells, "nchem": sc.nchem, "t_end_s": round(t_reached, 1),
+                   "loop_iterations": steps, "solver_steps": solver_steps},
         "roofline": roofline,
         "kernels": kernels,
         "step_roofline_frac": round(step_frac, 4),
