@@ -250,6 +250,7 @@ def main() -> int:
     # timed: the whole run, t = 0 .. t_end, device-resident between print times
     # (PRINT_STEP as in the reference; the host only reads the clock there)
     d32 = capi.HipDomain.from_scenario(sc, precision=capi.FP32_MIXED, ade=True, init=init)
+    tiling = d32.tiling()              # from the autotuner's table (fluxos_cpp_amd/tuning.json)
     v0 = d32.mass_balance()[0]
     if args.steps > 0:                  # only the first K iterations of the run
         info, el, kt = timed(d32, args.steps)
@@ -388,7 +389,7 @@ def main() -> int:
                                f"mm/day over 6 h, 2 solutes, FP32-mixed with fast ADE, "
                                + ("the whole 6 simulated hours" if args.steps == 0 else f"first {args.steps} iterations"),
                    "cells": cells, "nchem": sc.nchem, "t_end_s": round(t_reached, 1),
-                   "loop_iterations": steps, "solver_steps": solver_steps},
+                   "loop_iterations": steps, "solver_steps": solver_steps, "tiling_ch_ach": list(tiling)},
         "roofline": roofline,
         "kernels": kernels,
         "step_roofline_frac": round(step_frac, 4),

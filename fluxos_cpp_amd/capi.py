@@ -41,7 +41,7 @@ EXPORTS = ("fx_domain_create", "fx_domain_destroy", "fx_domain_bytes", "fx_set_s
            "fx_synchronize", "fx_read_geo", "fx_initiation", "fx_last_error", "fx_version",
            "fx_profile", "fx_profile_read", "fx_batch_create", "fx_batch_destroy", "fx_batch_admit",
            "fx_batch_resume", "fx_batch_evict", "fx_batch_run", "fx_batch_poll", "fx_batch_info",
-           "fx_read_dem", "fx_read_forcing", "fx_write_results")
+           "fx_read_dem", "fx_read_forcing", "fx_write_results", "fx_set_tiling", "fx_get_tiling")
 EV_PRINT, EV_DONE, EV_ERROR = 1, 2, 3
 K_CFL, K_MELT, K_FLUX, K_UPDATE, K_ADE_PASS1, K_ADE_SWEEP, K_STEP32, K_ADE32, K_CFL32 = range(9)
 KERNEL_NAMES = {K_CFL: "k64_cfl", K_MELT: "k64_melt", K_FLUX: "k64_flux", K_UPDATE: "k64_update",
@@ -120,6 +120,8 @@ def load(path: Path | None = None) -> C.CDLL:
         "fx_adesolver_calc": ([vp, C.c_int32, C.c_int32], C.c_int),
         "fx_advance": ([vp, C.c_double, C.c_double, C.c_int32, C.POINTER(StepInfo)], C.c_int),
         "fx_mass_balance": ([vp, _D], C.c_int),
+        "fx_set_tiling": ([vp, C.c_int32, C.c_int32], C.c_int),
+        "fx_get_tiling": ([vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
         "fx_synchronize": ([vp], C.c_int),
         "fx_read_geo": ([C.c_int32, C.c_int32, C.c_double, _F, C.c_double, _D, _D, _F], C.c_int),
         "fx_initiation": ([C.c_int32, C.c_int32, C.c_int32, _D, _D, _D, _F, _D], C.c_int),
@@ -375,13 +377,25 @@ class HipDomain:
         _check(self.lib.fx_profile_read(self.h, int(kernel), C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    def set_tiling(self, ch: int, ach: int) -> None:
+        """Column-block lengths of the FP32 sweeps (results do not depend on it)."""
+        _check(self.lib.fx_set_tiling(self.h, ch, ach))
+
+    def tiling(self) -> tuple[int, int]:
+        ch, ach = C.c_int32(), C.c_int32()
+        _check(self.lib.fx_get_tiling(self.h, C.byref(ch), C.byref(ach)))
+        return ch.value, ach.value
+
     def synchronize(self) -> None:
         _check(self.lib.fx_synchronize(self.h))
 
     # -- convenience -------------------------------------------------------------------------------
     @classmethod
     def from_scenario(cls, sc, precision: int = FP64_PARITY, ade: bool = True, ade_mode=None,
-                      init: InitialState | None = None, device: int = 0) -> "HipDomain":
+                      init: InitialState | None = None, device: int = 0, tiling="auto") -> "HipDomain":
+        """A device domain for a gen.Scenario.  FP32-mixed domains take the
+        tile shape the autotuner recorded for the nearest grid size
+        (tiling="auto"), an explicit (ch, ach), or the defaults (None)."""
         st = init or prepare(sc, ade=ade)
         d = cls(sc.nrows, sc.ncols, sc.dxy, sc.hdry, sc.nchem, sc.d_coef, ade, precision, ade_mode,
                 device=device)
@@ -392,7 +406,32 @@ class HipDomain:
         d.set_conc(0, st.conc0)
         d.set_meteo(sc.meteo, st.melt_mult)
         d.set_clock(0.0, 0.0, 0)
+        if precision == FP32_MIXED and tiling is not None:
+            if tiling == "auto":
+                from . import autotune
+                tiling = autotune.choose(sc.nrows, sc.ncols)
+            if tiling:
+                d.set_tiling(*tiling)
         return d
+
+    SNAPSHOT_FIELDS = ("z", "h", "qx", "qy", "fe1", "fe2", "fe3", "fn1", "fn2", "fn3", "twet", "ldry",
+                       "ldry_prev", "h0")
+
+    def snapshot(self, solutes: bool = True) -> dict:
+        """Every grid a run restarts from (reference layout) and the clock."""
+        out = {k: self.get(k) for k in self.SNAPSHOT_FIELDS}
+        out["conc"] = [self.get_conc(k) for k in range(self.nchem)] if solutes else []
+        out["clock"] = self.clock()
+        return out
+
+    def restore(self, snap: dict) -> None:
+        for k in self.SNAPSHOT_FIELDS:
+            if k in snap:
+                self.set(k, snap[k])
+        for k, c in enumerate(snap.get("conc", [])):
+            self.set_conc(k, c)
+        t, dt, it = snap["clock"]
+        self.set_clock(t, dt, it)
 
     def state(self) -> dict:
         out = {k: self.get(k) for k in ("z", "h", "qx", "qy", "ldry")}

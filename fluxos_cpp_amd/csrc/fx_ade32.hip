@@ -34,7 +34,6 @@
 #pragma clang fp contract(fast)
 
 #define AOWN FX32_AOWN        // rows owned per strip
-#define ACH FX32_ACH          // columns per block
 #define ANC 4                 // solutes per launch
 
 namespace {
@@ -267,8 +266,8 @@ __device__ inline void ade_tile(const fx_params &P, const a32p &A, fx_ctl *ctl, 
     }
     const int lane = threadIdx.x;
     const int r0 = 1 + bx * AOWN;
-    const int c0 = 1 + by * ACH;
-    const int c_last = min(c0 + ACH - 1, P.ncols);
+    const int c0 = 1 + by * P.ach;
+    const int c_last = min(c0 + P.ach - 1, P.ncols);
     X.row = r0 - 3 + lane;
     X.row_ok = X.row >= 0 && X.row < P.ld;
     X.mcols = P.ncols + 2;
@@ -338,7 +337,7 @@ void launch_nc(fx_domain d, const fx_params &P, a32p A, dim3 g)
 
 inline dim3 ade_grid(const fx_params &P)
 {
-    return dim3((P.nrows + AOWN - 1) / AOWN, (P.ncols + ACH - 1) / ACH);
+    return dim3((P.nrows + AOWN - 1) / AOWN, (P.ncols + P.ach - 1) / P.ach);
 }
 
 } // namespace
@@ -361,6 +360,12 @@ a32p make_a32p(fx_domain d)
 int fx32_ade_gx(fx_domain d)
 {
     return (int)ade_grid(fx_make_params(d)).x;
+}
+
+int fx32_ade_blocks(fx_domain d)
+{
+    const dim3 g = ade_grid(fx_make_params(d));
+    return (int)(g.x * g.y);
 }
 
 int fx32b_launch_ade(hipStream_t st, const fx_bslot *slots, const uint32_t *amap, int nblocks, int nchem)

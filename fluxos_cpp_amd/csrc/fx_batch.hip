@@ -39,6 +39,7 @@ struct fx_batch_s {
     std::vector<fx_domain> member;     // nullptr = free slot
     std::vector<char> reported;        // halt already reported by poll
     std::vector<double> t_end;
+    std::vector<int> nab;              // ADE blocks of each member
     fx_bstat *stat_dev = nullptr;
     fx_bstat *stat_host = nullptr;     // pinned
     uint32_t *tmap = nullptr, *amap = nullptr;
@@ -88,10 +89,8 @@ int rebuild(fx_batch b)
         b->nslot = s + 1;
         const fx_bslot &B = b->slots_host[s];
         for (int k = 0; k < B.npart; k++) t.push_back(FX_BMAP(s, k));
-        if (B.ade_on) {
-            const int nab = B.agx * ((B.P.ncols + FX32_ACH - 1) / FX32_ACH);
-            for (int k = 0; k < nab; k++) a.push_back(FX_BMAP(s, k));
-        }
+        if (B.ade_on)
+            for (int k = 0; k < b->nab[s]; k++) a.push_back(FX_BMAP(s, k));
     }
     int rc;
     if ((rc = grow(&b->tmap, &b->tmap_cap, std::max<int64_t>(1, (int64_t)t.size())))) return rc;
@@ -170,6 +169,7 @@ int fx_batch_create(int32_t device, int32_t nchem, int32_t ade_solver, int32_t m
     b->member.assign(max_members, nullptr);
     b->reported.assign(max_members, 0);
     b->t_end.assign(max_members, 0.0);
+    b->nab.assign(max_members, 0);
     auto bail = [&](hipError_t e, const char *what) {
         int rc = fx_hip_check(e, what);
         fx_batch_destroy(b);
@@ -251,7 +251,9 @@ int fx_batch_admit(fx_batch b, fx_domain d, double t_end, double print_next, int
     B.gx = (B.P.nrows + FX32_TH - 1) / FX32_TH;
     B.agx = fx32_ade_gx(d);
     B.ade_on = dade;
+    b->nab[s] = fx32_ade_blocks(d);
     b->member[s] = d;
+    d->in_batch = true;
     b->reported[s] = 0;
     b->t_end[s] = t_end;
     b->stat_host[s] = fx_bstat{c->tim, c->dtfl, c->it, 0, 0, c->halt, c->status, 0};
@@ -287,6 +289,7 @@ int fx_batch_evict(fx_batch b, int32_t slot)
     int rc;
     if ((rc = set_dev(b))) return rc;
     FX_HIP(hipStreamSynchronize(b->stream));
+    b->member[slot]->in_batch = false;
     b->member[slot] = nullptr;
     b->slots_host[slot] = fx_bslot{};
     b->slots_host[slot].ctl = b->idle_ctl;

@@ -11,7 +11,7 @@
 #include <string>
 #include <vector>
 
-#include "fx_params.h"
+#include "fx_kern32.h"
 
 static thread_local std::string g_err;
 
@@ -92,6 +92,8 @@ fx_params fx_make_params(fx_domain d)
     P.inflow_ncols = d->inflow_ncols;
     P.inflow_p = d->inflow ? d->inflow_irow + d->inflow_icol * (int64_t)d->mrows : -1;
     P.rates_nonneg = d->rates_nonneg ? 1 : 0;
+    P.ch = d->ch;
+    P.ach = d->ach;
     P.dxy = (double)D.dxy;
     P.arbase = (double)((int64_t)D.dxy * (int64_t)D.dxy);
     P.dxy_u = (uint32_t)D.dxy;
@@ -282,6 +284,9 @@ int fx_domain_create(const fx_domain_desc *desc, fx_domain *out)
     d->ncell = (int64_t)d->mrows * d->mcols;
     d->rates_nonneg = true;
     d->graph_ok = false;
+    d->ch = FX32_CH;
+    d->ach = FX32_ACH;
+    d->in_batch = false;
     auto bail = [&](int code) { fx_domain_destroy(d); return code; };
     if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fx_fail(FX_ERR_HIP, "hipStreamCreate failed"));
@@ -321,6 +326,7 @@ int fx_domain_create(const fx_domain_desc *desc, fx_domain *out)
 
 int fx_domain_destroy(fx_domain d)
 {
+    if (d && d->in_batch) return fx_fail(FX_ERR_STATE, "domain is a batch member: evict it first");
     if (!d) return FX_OK;
     (void)hipSetDevice(d->desc.device);
     if (d->stream) (void)hipStreamSynchronize(d->stream);
@@ -655,6 +661,25 @@ int fx_advance(fx_domain d, double t_end, double print_next, int32_t max_steps, 
     info->status = (c->tim >= print_next ? 1 : 0) | (c->tim > t_end ? 2 : 0) | (c->status & 4);
     if (c->ade_error) return fx_fail(FX_ERR_HIP, "ADE wavefront wait expired");
     if (c->status & 4) return fx_fail(FX_ERR_NONFINITE, "non-finite value in the state");
+    return FX_OK;
+}
+
+int fx_set_tiling(fx_domain d, int32_t ch, int32_t ach)
+{
+    int rc = enter(d);
+    if (rc) return rc;
+    if (d->desc.precision != FX_FP32_MIXED) return fx_fail(FX_ERR_UNSUPPORTED, "tiling applies to FP32-mixed domains");
+    if (d->in_batch) return fx_fail(FX_ERR_STATE, "domain is a batch member: its tiling is fixed");
+    if (ch < FX32_CH_MIN || ch > 4096 || ach < FX32_CH_MIN || ach > 4096)
+        return fx_fail(FX_ERR_ARG, "column-block lengths must be 8..4096");
+    return fx32_set_tiling(d, ch, ach);
+}
+
+int fx_get_tiling(fx_domain d, int32_t *ch, int32_t *ach)
+{
+    if (!d) return fx_fail(FX_ERR_ARG, "domain is NULL");
+    if (ch) *ch = d->ch;
+    if (ach) *ach = d->ach;
     return FX_OK;
 }
 

@@ -267,7 +267,6 @@ __device__ inline void cfl_cell(const fx_params &P, double h, float qx, float qy
 // shared with the neighbouring block.
 // ---------------------------------------------------------------------------
 #define TH FX32_TH
-#define CH FX32_CH
 #define NS 64                 // rows held per column: r0-2 .. r0+61
 
 struct pref {                 // one cell's HBM values in flight
@@ -504,8 +503,8 @@ __device__ inline void sweep_tile(const fx_params &P, const s32p &S, fx_ctl *ctl
     X.s_wet_prev = ctl->s_wet_prev;
     X.k = threadIdx.x;
     const int r0 = 1 + bx * TH;
-    const int c0 = 1 + by * CH;
-    const int c_last = min(c0 + CH - 1, P.ncols);
+    const int c0 = 1 + by * P.ch;
+    const int c_last = min(c0 + P.ch - 1, P.ncols);
     X.row = r0 - 2 + X.k;                             // this lane's row
     X.row_ok = X.row >= 0 && X.row < P.ld;
     X.flux_lane = X.k >= 1 && X.k <= NS - 2;
@@ -578,11 +577,11 @@ __global__ __launch_bounds__(64) void k32_cfl(fx_params P, s32p S, fx_ctl *ctl)
     const double *d_in = S.d_in[cur];
     const float *qx_in = S.qx_in[cur], *qy_in = S.qy_in[cur];
     const int k = threadIdx.x;
-    const int r0 = 1 + blockIdx.x * TH, c0 = 1 + blockIdx.y * CH;
+    const int r0 = 1 + blockIdx.x * TH, c0 = 1 + blockIdx.y * P.ch;
     const int row = r0 - 2 + k;
     double dtm = 9.e10, hmx = 0.0;
     if (k >= 2 && k <= NS - 2 && row <= P.nrows) {
-        for (int c = c0; c <= min(c0 + CH - 1, P.ncols); c++) {
+        for (int c = c0; c <= min(c0 + P.ch - 1, P.ncols); c++) {
             const uint32_t p = (uint32_t)row + (uint32_t)c * (uint32_t)P.ld;
             cfl_cell(P, fmax(0.0, d_in[p]), qx_in[p], qy_in[p], dtm, hmx);
         }
@@ -699,11 +698,11 @@ __global__ __launch_bounds__(64) void k32_volume(fx_params P, s32p S, fx_ctl *ct
 {
     const double *d_in = S.d_in[ctl->cur];
     const int k = threadIdx.x;
-    const int r0 = 1 + blockIdx.x * TH, c0 = 1 + blockIdx.y * CH;
+    const int r0 = 1 + blockIdx.x * TH, c0 = 1 + blockIdx.y * P.ch;
     const int row = r0 - 2 + k;
     double v = 0.0;
     if (k >= 2 && k <= NS - 2 && row <= P.nrows) {
-        for (int c = c0; c <= min(c0 + CH - 1, P.ncols); c++) {
+        for (int c = c0; c <= min(c0 + P.ch - 1, P.ncols); c++) {
             const uint32_t p = (uint32_t)row + (uint32_t)c * (uint32_t)P.ld;
             if (!(S.flags[p] & FXF_WEIR)) v += d_in[p];
         }
@@ -714,7 +713,7 @@ __global__ __launch_bounds__(64) void k32_volume(fx_params P, s32p S, fx_ctl *ct
 
 inline dim3 grid32(const fx_params &P)
 {
-    return dim3((P.nrows + TH - 1) / TH, (P.ncols + CH - 1) / CH);
+    return dim3((P.nrows + TH - 1) / TH, (P.ncols + P.ch - 1) / P.ch);
 }
 
 } // namespace
@@ -811,7 +810,9 @@ int fx32_init_state(fx_domain d)
     fx_params P = fx_make_params(d);
     dim3 g = grid32(P);
     t.npart = (int64_t)g.x * g.y;
-    if ((rc = dalloc32(d, &t.partials, 3 * t.npart))) return rc;
+    // room for the partials of the smallest column block fx_set_tiling allows
+    const int64_t cap = (int64_t)g.x * ((P.ncols + FX32_CH_MIN - 1) / FX32_CH_MIN);
+    if ((rc = dalloc32(d, &t.partials, 3 * cap))) return rc;
     return FX_OK;
 }
 
@@ -1080,5 +1081,32 @@ int fx32_pending_weir(fx_domain d, double *vol)
     double s = 0.0;
     for (size_t i = 0; i < nb; i++) s += h[3 * i + 2];
     *vol = s * (double)d->desc.dxy * (double)d->desc.dxy;
+    return FX_OK;
+}
+
+// change the column-block lengths of the FP32 sweeps.  The pending weir-face
+// exchange of the last step is held in the old tiling's partials: it is folded
+// into the FP64 ledger first, and the partials are cleared.
+int fx32_set_tiling(fx_domain d, int ch, int ach)
+{
+    if (ch == d->ch && ach == d->ach) return FX_OK;
+    double pend = 0.0;
+    int rc;
+    if ((rc = fx32_pending_weir(d, &pend))) return rc;
+    FX_HIP(hipMemcpyAsync(d->ctl_host, d->ctl, sizeof(fx_ctl), hipMemcpyDeviceToHost, d->stream));
+    FX_HIP(hipStreamSynchronize(d->stream));
+    fx_ctl *c = d->ctl_host;
+    const double y = pend - c->led_weir_c;
+    const double t = c->led_weir + y;
+    c->led_weir_c = (t - c->led_weir) - y;
+    c->led_weir = t;
+    FX_HIP(hipMemcpyAsync(d->ctl, c, sizeof(fx_ctl), hipMemcpyHostToDevice, d->stream));
+    FX_HIP(hipMemsetAsync(d->s32.partials, 0, 3 * (size_t)d->s32.npart * sizeof(double), d->stream));
+    d->ch = ch;
+    d->ach = ach;
+    fx_params P = fx_make_params(d);
+    const dim3 g = grid32(P);
+    d->s32.npart = (int64_t)g.x * g.y;
+    FX_HIP(hipStreamSynchronize(d->stream));
     return FX_OK;
 }

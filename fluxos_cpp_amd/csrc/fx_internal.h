@@ -119,6 +119,8 @@ struct fx_domain_s {
     std::vector<double> zb_host;   // FP64 bed (FP32 mode output conversion)
     hipGraphExec_t graph;      // one captured iteration of the loop
     bool graph_ok;
+    int32_t ch, ach;           // FP32 sweep column-block lengths (fx_set_tiling)
+    bool in_batch;             // member of a batch: tiling is fixed
     int64_t bytes;
     fx_prof prof;
 };

@@ -31,9 +31,10 @@ struct a32p {
 
 // tiling of the FP32 kernels
 #define FX32_TH 61            // rows owned per step-sweep strip
-#define FX32_CH 32            // columns per step-sweep block
+#define FX32_CH 32            // default columns per step-sweep block
+#define FX32_CH_MIN 8         // smallest step-sweep block (sizes the CFL partials)
 #define FX32_AOWN 58          // rows owned per ADE-sweep strip
-#define FX32_ACH 64           // columns per ADE-sweep block
+#define FX32_ACH 64           // default columns per ADE-sweep block
 
 // One member of a batch, in device memory: everything its kernels need.
 struct fx_bslot {
@@ -58,6 +59,8 @@ s32p make_s32p(fx_domain d);
 a32p make_a32p(fx_domain d);
 int fx32_npart(fx_domain d);
 int fx32_ade_gx(fx_domain d);
+int fx32_ade_blocks(fx_domain d);
+int fx32_set_tiling(fx_domain d, int ch, int ach);
 
 // batched launches over nslot members; tmap / amap give, per block, the
 // member (high 12 bits) and the member's own block index (low 20 bits)

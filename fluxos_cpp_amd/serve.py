@@ -82,11 +82,13 @@ def run_ensemble(scens: list[gen.Scenario], inits: list[capi.InitialState], ade:
                  steps_per_poll: int = 16) -> dict:
     """Admit every scenario into one batch and run all to their end time.
     The timed region starts with every input resident on the device."""
+    from . import autotune
     nchem = max(sc.nchem for sc in scens)
     batch = capi.Batch(nchem=nchem, ade=ade, max_members=len(scens))
     doms, t_end, next_print = {}, {}, {}
+    tiling = autotune.choose_cells(sum(sc.nrows * sc.ncols for sc in scens))
     for sc, st in zip(scens, inits):
-        d = capi.HipDomain.from_scenario(sc, precision=capi.FP32_MIXED, ade=ade, init=st)
+        d = capi.HipDomain.from_scenario(sc, precision=capi.FP32_MIXED, ade=ade, init=st, tiling=tiling)
         s = batch.admit(d, float(sc.t_end), float(sc.print_step))
         doms[s], t_end[s], next_print[s] = d, float(sc.t_end), float(sc.print_step)
     done: dict[int, int] = {}
@@ -97,7 +99,7 @@ def run_ensemble(scens: list[gen.Scenario], inits: list[capi.InitialState], ade:
                lambda s, tim, steps, solver: done.__setitem__(s, steps), steps_per_poll)
     el = time.perf_counter() - t0
     n, cells, upd = batch.info()
-    out = {"scenarios": n, "cells": cells, "cell_updates": upd, "wall_s": el,
+    out = {"scenarios": n, "cells": cells, "cell_updates": upd, "wall_s": el, "tiling": tiling,
            "cell_updates_per_s": upd / el, "max_steps": max(done.values()),
            "min_steps": min(done.values())}
     batch.close()
@@ -147,6 +149,7 @@ def serve(requests: list[Request], budget_bytes: int = 200 * 10 ** 9, max_member
           steps_per_poll: int = 8, keep_results: bool = False) -> dict:
     """Continuous batching of a request stream (arrival times relative to the
     start).  Returns latency percentiles and throughput."""
+    from . import autotune
     pending = deque(sorted(requests, key=lambda r: r.arrival))
     batch = capi.Batch(nchem=1, ade=True, max_members=max_members)
     live: dict[int, tuple[Request, capi.HipDomain, int]] = {}
@@ -183,8 +186,10 @@ def serve(requests: list[Request], budget_bytes: int = 200 * 10 ** 9, max_member
             if used + nb > budget_bytes or len(live) >= max_members:
                 break
             pending.popleft()
+            # tile shape for how full the device will be with this member
+            live_cells = sum(r.cells for r, _, _ in live.values()) + req.cells
             dom = capi.HipDomain.from_scenario(req.scenario, precision=capi.FP32_MIXED, ade=True,
-                                               init=req.init)
+                                               init=req.init, tiling=autotune.choose_cells(live_cells))
             slot = batch.admit(dom, float(req.scenario.t_end), float(req.scenario.print_step))
             live[slot] = (req, dom, nb)
             t_end[slot] = float(req.scenario.t_end)

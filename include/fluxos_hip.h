@@ -153,6 +153,12 @@ int fx_advance(fx_domain dom, double t_end, double print_next, int32_t max_steps
  * weir-face exchange, all accumulated in FP64 (extension; no counterpart in
  * the reference).  out[0]=volume, out[1]=melt in, out[2]=net weir inflow. */
 int fx_mass_balance(fx_domain dom, double out[3]);
+/* Tile shape of the FP32-mixed sweeps (extension; no counterpart in the
+ * reference): columns walked per block by the step sweep (ch) and by the
+ * ADE sweep (ach), 8..4096; defaults 32 and 64.  Results do not depend on it.
+ * Chosen per grid size by the autotuner (fluxos_cpp_amd/autotune.py).     */
+int fx_set_tiling(fx_domain dom, int32_t ch, int32_t ach);
+int fx_get_tiling(fx_domain dom, int32_t *ch, int32_t *ach);
 int fx_synchronize(fx_domain dom);
 
 /* ---- batches of scenarios (extension; no counterpart in the reference) ------
