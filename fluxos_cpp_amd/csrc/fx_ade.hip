@@ -58,6 +58,51 @@ __device__ inline double shift_up(double v)
     return __shfl_up(v, 1, ADE_W);
 }
 
+// Inputs of one row of a lane's column that do not depend on the sweep's
+// carried values, so they are loaded ADE_DEPTH rows ahead (row r, column iy):
+struct arow {
+    double Bee;     // B(min(r+2, NROWS+1), iy): pass-1 value two rows on
+    double Ae;      // A(r+1, iy): raw value of the E cell (ghost row at r = NROWS)
+    double he;      // h(r+1, iy)
+    double qx, qy;  // fe_1(r, iy), fn_1(r, iy)
+    double Bs;      // B(r, iy-1)
+    double An, Bn;  // A(r, iy+1) raw (ghost column at iy = NCOLS), B(r, iy+1)
+    double hn;      // h(r, iy+1)
+    double Bnn;     // B(r, min(iy+2, NCOLS+1))
+    uint32_t dry;   // ldry bits: 1 = EE, 2 = S, 4 = N, 8 = NN
+};
+
+struct actx64 {
+    const double *A, *B, *h, *fe1, *fn1;
+    const uint8_t *ldry;
+    size_t ld;
+    int nr, nc, iy, inn;
+    bool col_ok;
+};
+
+__device__ inline void arow_load(const actx64 &X, int r, arow &a)
+{
+    if (!X.col_ok || r < 1 || r > X.nr) return;
+    const size_t ld = X.ld, iy = (size_t)X.iy;
+    const size_t p = (size_t)r + iy * ld;
+    const size_t iee = (size_t)((r + 2 < X.nr + 1) ? r + 2 : X.nr + 1);
+    a.Bee = X.B[iee + iy * ld];
+    a.Ae = X.A[p + 1];
+    a.he = X.h[p + 1];
+    a.qx = X.fe1[p];
+    a.qy = X.fn1[p];
+    a.Bs = X.B[p - ld];
+    a.An = X.A[p + ld];
+    a.Bn = X.B[p + ld];
+    a.hn = X.h[p + ld];
+    const size_t pnn = (size_t)r + (size_t)X.inn * ld;
+    a.Bnn = X.B[pnn];
+    a.dry = (X.ldry[iee + iy * ld] == 1 ? 1u : 0u) | (X.ldry[p - ld] == 1 ? 2u : 0u) |
+            (X.ldry[p + ld] == 1 ? 4u : 0u) | (X.ldry[pnn] == 1 ? 8u : 0u);
+}
+
+#define ADE_DEPTH 4
+
 __global__ __launch_bounds__(ADE_W) void k_ade_sweep(fx_params P, fx_state64 s, int ichem,
                                                      fx_ctl *ctl, unsigned mode,
                                                      uint32_t *progress, double *qedge)
@@ -73,22 +118,45 @@ __global__ __launch_bounds__(ADE_W) void k_ade_sweep(fx_params P, fx_state64 s, 
     const int iy = b * ADE_W + 1 + lane;
     const bool col_ok = iy <= nc;
     double *A = s.conc[ichem];
-    const double *B = s.conc_tmp;
-    const double *fe1 = s.fe1, *fn1 = s.fn1, *h = s.h;
     const uint8_t *ldry = s.ldry;
     const double dx = P.dxy, dy = P.dxy, area = P.arbase, areae = area, arean = area;
     const double dtfl = ctl->dtfl, D = P.D[ichem];
     const double nt = 1, sigc = 0.5;
-    const int is = iy - 1, in = iy + 1, inn = (iy + 2 < nc + 1) ? iy + 2 : nc + 1;
+    actx64 X;
+    X.A = A; X.B = s.conc_tmp; X.h = s.h; X.fe1 = s.fe1; X.fn1 = s.fn1; X.ldry = ldry;
+    X.ld = ld; X.nr = nr; X.nc = nc; X.iy = iy; X.col_ok = col_ok;
+    X.inn = (iy + 2 < nc + 1) ? iy + 2 : nc + 1;
 
     double pfe = 0.0;          // east-face flux carried along the column
     double fw_final = 0.0;     // final concentration of the W cell
     double q_out = 0.0, c_out = 0.0;   // this lane's result of the last iteration
     uint32_t seen = 0;         // producer progress already observed (lane 0)
     bool bail = false;
+    // sliding window of the lane's own column: pass-1 values B, wet flags and
+    // depth at rows ix-1, ix, ix+1 (the row ix+2 values arrive with each row)
+    double Bw = 0.0, Bp = 0.0, Be = 0.0, hp_w = 0.0;
+    uint32_t dw = 0, dp = 0, de = 0;
+    double a0 = 0.0, f0 = 0.0;  // ghost row 0 of the column: raw value, fe_1
+    if (col_ok) {
+        const size_t c0 = (size_t)iy * ld;
+        Bw = X.B[c0];
+        Bp = X.B[c0 + 1];
+        Be = X.B[c0 + (nr >= 2 ? 2 : 1)];
+        hp_w = X.h[c0 + 1];
+        dw = ldry[c0] == 1;
+        dp = ldry[c0 + 1] == 1;
+        de = ldry[c0 + (nr >= 2 ? 2 : 1)] == 1;
+        a0 = A[c0];
+        f0 = X.fe1[c0];
+    }
+    arow ring[ADE_DEPTH];
+    for (int k = 0; k < ADE_DEPTH; k++) {      // rows of steps 0 .. ADE_DEPTH-1
+        ring[k] = arow{};
+        arow_load(X, k - lane + 1, ring[k]);
+    }
 
     const int T = nr + ADE_W - 1;
-    for (int t = 0; t < T; t++) {
+    auto step = [&](int t, arow &R) {
         // receive the S neighbour's (qfn, final c) for this lane's row
         double q_in = shift_up(q_out), c_in = shift_up(c_out);
         const int ix = t - lane + 1;
@@ -114,84 +182,79 @@ __global__ __launch_bounds__(ADE_W) void k_ade_sweep(fx_params P, fx_state64 s, 
                                          __HIP_MEMORY_SCOPE_AGENT);
             }
         }
-        if (__shfl(bail ? 1 : 0, 0, ADE_W)) {
-            if (lane == 0) atomicOr(&ctl->ade_error, 1u);
-            return;
-        }
+        if (__shfl(bail ? 1 : 0, 0, ADE_W)) return false;
         if (active) {
             const size_t p = (size_t)ix + (size_t)iy * ld;
+            const double Bee = R.Bee;
+            const uint32_t dee = R.dry & 1u;
             const bool last_cell = (ix == nr && iy == nc);   // a < NCOLS*NROWS
             if (last_cell) {
                 q_out = 0.0;
-                c_out = B[p];
+                c_out = Bp;
                 A[p] = c_out;
             } else {
-                const int iw = ix - 1, ie = ix + 1, iee = (ix + 2 < nr + 1) ? ix + 2 : nr + 1;
-                if (ix == 1) pfe = A[(size_t)iy * ld] * fe1[(size_t)iy * ld] * dy;   // :106-116
-                if (ix == 1) fw_final = A[(size_t)iy * ld];
-                if (ldry[p] == 1) {                                                  // :119-124
+                if (ix == 1) pfe = a0 * f0 * dy;                                     // :106-116
+                if (ix == 1) fw_final = a0;
+                if (dp) {                                                            // :119-124
                     pfe = 0.0;
                     q_out = 0.0;
                     c_out = 0.0;
                 } else {
-                    const size_t pE = p + 1, pEE = (size_t)iee + (size_t)iy * ld;
-                    const size_t pW = p - 1, pS = p - ld, pN = p + ld;
-                    const size_t pNN = (size_t)ix + (size_t)inn * ld;
                     // neighbour bounds of pass 1: W,S rescaled, E,N raw (:51-62)
-                    const double cmaxr = fmax(B[pW], fmax(A[pE], fmax(B[pS], A[pN])));
-                    const double cminr = fmin(B[pW], fmin(A[pE], fmin(B[pS], A[pN])));
-                    const double hp = h[p], he = h[pE], hn = h[pN];
-                    const double qxl = fe1[p], qyl = fn1[p];
-                    const double fw = fw_final, fp = B[p], fe = B[pE], fee = B[pEE];
-                    const double fs = c_in, fn = B[pN], fnn = B[pNN];
+                    const double cmaxr = fmax(Bw, fmax(R.Ae, fmax(R.Bs, R.An)));
+                    const double cminr = fmin(Bw, fmin(R.Ae, fmin(R.Bs, R.An)));
+                    const double hp = hp_w, he = R.he, hn = R.hn;
+                    const double qxl = R.qx, qyl = R.qy;
+                    const double fw = fw_final, fp = Bp, fe = Be, fee = Bee;
+                    const double fs = c_in, fn = R.Bn, fnn = R.Bnn;
                     const double pfw = pfe, qfs = q_in;
                     double fem, pfde, fnm, qfdn;
-                    if (ldry[pE] == 0) {
+                    if (!de) {
                         const double hnue = fmax(hp * nt * he * nt, .0001);
                         const double hne = sqrt(hnue) / sigc / dx * dy * D;
                         pfde = -hne * (fe - fp);
                         if (qxl > 0.0) {
-                            if (ldry[pW] == 0) fem = -.125 * fw + .75 * fp + .375 * fe;
+                            if (!dw) fem = -.125 * fw + .75 * fp + .375 * fe;
                             else fem = 0.5 * fp + 0.5 * fe;
                         } else {
-                            if (ldry[pEE] == 0) fem = .375 * fp + .75 * fe - .125 * fee;
+                            if (!dee) fem = .375 * fp + .75 * fe - .125 * fee;
                             else fem = 0.5 * fp + 0.5 * fe;
                         }
                     } else {
                         fem = 0.; pfde = 0.;
                     }
                     fem = fmax(0., fem);
-                    if (ix == nr) fem = A[(size_t)(nr + 1) + (size_t)iy * ld];
+                    if (ix == nr) fem = R.Ae;                       // A(NROWS+1, iy)
                     const double pfce = qxl * fem * dy;
                     pfe = pfce + pfde;
                     if (pfe < 0) {
-                        if (ldry[pE] == 0) {
+                        if (!de) {
                             const double cvolrate = -(fe * he) * areae / dtfl;
                             pfe = fmax(pfe, cvolrate);
                         } else {
                             pfe = 0.;
                         }
                     }
-                    if (ldry[pN] == 0) {
+                    if (!(R.dry & 4u)) {
                         const double hnue = fmax(.0001, hp * nt * hn * nt);
                         const double hnn = sqrt(hnue) / sigc / dy * dx * D;
                         qfdn = -hnn * (fn - fp);
                         if (qyl > 0.0) {
-                            if (ldry[pS] == 0) fnm = -.125 * fs + .75 * fp + .375 * fn;
+                            if (!(R.dry & 2u)) fnm = -.125 * fs + .75 * fp + .375 * fn;
                             else fnm = 0.5 * fp + .5 * fn;
                         } else {
-                            if (ldry[pNN] == 0) fnm = .375 * fp + .75 * fn - .125 * fnn;
+                            if (!(R.dry & 8u)) fnm = .375 * fp + .75 * fn - .125 * fnn;
                             else fnm = .5 * fp + .5 * fn;
                         }
                     } else {
                         fnm = 0.; qfdn = 0.;
                     }
                     fnm = fmax(0., fnm);
-                    if (iy == nc) fnm = A[(size_t)ix + (size_t)(nc + 1) * ld];
+                    if (iy == nc) fnm = R.An;                       // A(ix, NCOLS+1)
                     const double qfcn = qyl * fnm * dx;
                     double qfn = qfcn + qfdn;
                     if (qfn < 0) {
-                        if (ldry[pN] == 0) {
+                        if (!(R.dry & 4u)) {
                             const double cvolrate = -(fn * hn) * arean / dtfl;
                             qfn = fmax(qfn, cvolrate);
                         } else {
@@ -233,16 +296,37 @@ __global__ __launch_bounds__(ADE_W) void k_ade_sweep(fx_params P, fx_state64 s, 
                     c_out = con;
                 }
                 A[p] = c_out;
-                (void)iw;
             }
             fw_final = c_out;
             if (lane == ADE_W - 1) qedge[(size_t)b * ld + ix] = q_out;
+            // slide the column window down one row
+            Bw = Bp; Bp = Be; Be = Bee;
+            dw = dp; dp = de; de = dee;
+            hp_w = R.he;
         }
         // publish the last lane's progress every few rows and at the end
         if (lane == ADE_W - 1 && active && ((ix % ADE_PUBLISH) == 0 || ix == nr)) {
             __threadfence();
             __hip_atomic_store(&progress[b], (uint32_t)ix, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
+        // this slot now fetches the row of step t + ADE_DEPTH
+        arow_load(X, t + ADE_DEPTH - lane + 1, R);
+        return true;
+    };
+    bool ok = true;
+    int t = 0;
+    for (; ok && t + ADE_DEPTH <= T; t += ADE_DEPTH) {
+        ok = step(t, ring[0]);
+        if (ok) ok = step(t + 1, ring[1]);
+        if (ok) ok = step(t + 2, ring[2]);
+        if (ok) ok = step(t + 3, ring[3]);
+    }
+    if (ok && t < T) ok = step(t, ring[0]);
+    if (ok && t + 1 < T) ok = step(t + 1, ring[1]);
+    if (ok && t + 2 < T) ok = step(t + 2, ring[2]);
+    if (!ok) {
+        if (lane == 0) atomicOr(&ctl->ade_error, 1u);
+        return;
     }
     // a block whose last lane is past NCOLS never publishes from the loop
     if (lane == 0 && !(iy + ADE_W - 1 <= nc)) {
