@@ -196,6 +196,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=0,
                     help="timed loop iterations (0 = the whole run, t = 0 .. t_end)")
     ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--start-t", type=float, default=0.0,
+                    help="with --steps K: spin up (untimed) to this simulated time first")
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--fp64-steps", type=int, default=20)
     ap.add_argument("--spinup-t", type=float, default=3600.0)
@@ -252,7 +254,10 @@ def main() -> int:
     d32 = capi.HipDomain.from_scenario(sc, precision=capi.FP32_MIXED, ade=True, init=init)
     tiling = d32.tiling()              # from the autotuner's table (fluxos_cpp_amd/tuning.json)
     v0 = d32.mass_balance()[0]
-    if args.steps > 0:                  # only the first K iterations of the run
+    if args.steps > 0:                  # K iterations, from t = 0 or from --start-t
+        if args.start_t > 0:
+            t_sp, n_sp = spin_up(d32, args.start_t, 10 ** 7)
+            log(f"[bench] untimed spin-up to t={t_sp:.1f}s in {n_sp} steps")
         info, el, kt = timed(d32, args.steps)
         steps, solver_steps, t_reached = info.steps, info.solver_steps, info.tim
     else:
@@ -387,7 +392,8 @@ def main() -> int:
         "dtype": "f32 (f64 depth, dt, mass)", "data": "synthetic",
         "config": {"workload": f"single domain {args.n}x{args.n}, dx=2 m, fractal DEM + 0.3% slope, melt 0-60 "
                                f"mm/day over 6 h, 2 solutes, FP32-mixed with fast ADE, "
-                               + ("the whole 6 simulated hours" if args.steps == 0 else f"first {args.steps} iterations"),
+                               + ("the whole 6 simulated hours" if args.steps == 0 else
+                                  f"{args.steps} iterations from t={args.start_t:.0f}s"),
                    "cells": cells, "nchem": sc.nchem, "t_end_s": round(t_reached, 1),
                    "loop_iterations": steps, "solver_steps": solver_steps, "tiling_ch_ach": list(tiling)},
         "roofline": roofline,
