@@ -196,6 +196,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=0,
                     help="timed loop iterations (0 = the whole run, t = 0 .. t_end)")
     ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--tiling", type=str, default="",
+                    help="ch,ach column-block lengths (default: the autotuner's table)")
     ap.add_argument("--start-t", type=float, default=0.0,
                     help="with --steps K: spin up (untimed) to this simulated time first")
     ap.add_argument("--n", type=int, default=4096)
@@ -252,6 +254,8 @@ def main() -> int:
     # timed: the whole run, t = 0 .. t_end, device-resident between print times
     # (PRINT_STEP as in the reference; the host only reads the clock there)
     d32 = capi.HipDomain.from_scenario(sc, precision=capi.FP32_MIXED, ade=True, init=init)
+    if args.tiling:
+        d32.set_tiling(*(int(x) for x in args.tiling.split(",")))
     tiling = d32.tiling()              # from the autotuner's table (fluxos_cpp_amd/tuning.json)
     v0 = d32.mass_balance()[0]
     if args.steps > 0:                  # K iterations, from t = 0 or from --start-t

@@ -247,9 +247,7 @@ int launch_step64(fx_domain d)
 int launch_step(fx_domain d)
 {
     if (d->desc.precision == FX_FP64_PARITY) return launch_step64(d);
-    int rc = fx32_launch_step(d);
-    if (rc) return rc;
-    return fx_launch_step_end(d);
+    return fx32_launch_step(d);                    // ends with the flip and halt
 }
 
 } // namespace
@@ -338,7 +336,7 @@ int fx_domain_destroy(fx_domain d)
     for (int k = 0; k < FX_MAX_CHEM; k++) dfree(s.conc[k]);
     fx_state32 &t = d->s32;
     void *ptrs32[] = {t.d[0], t.d[1], t.qx[0], t.qx[1], t.qy[0], t.qy[1], t.zb, t.zb_lo, t.f6, t.ff,
-                      t.flags, t.tw_acc, t.tw_t0, t.partials};
+                      t.flags, t.tw_acc, t.tw_t0, t.partials, t.part2, t.ticket};
     for (void *p : ptrs32) dfree(p);
     for (int k = 0; k < FX_MAX_CHEM; k++) { dfree(t.mass[0][k]); dfree(t.mass[1][k]); }
     for (auto &r : d->prof.pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }

@@ -598,16 +598,14 @@ __global__ __launch_bounds__(64) void k32_cfl(fx_params P, s32p S, fx_ctl *ctl)
 // Clock of one loop iteration (main.cpp:271-325, 385-387): fold the CFL
 // partials left by the previous step, clip to the print time, advance tim,
 // evaluate the forcing, gate the solver, update the FP64 water ledger.
-__device__ inline void clock_body(const fx_params &P, double *partials, int nblk, fx_ctl *ctl)
+//
+// clock_fold: one block's min dt / max hpall / weir-volume sum over partials
+// [lo, hi), in a fixed order (the weir entries are consumed: set to 0).
+__device__ inline void clock_fold(double *partials, int lo, int hi, double &a, double &b, double &c)
 {
-    if (ctl->halt) return;
     const int tid = threadIdx.x;
-    if (ctl->tim > ctl->t_end) {                       // loop condition, main.cpp:271
-        if (tid == 0) ctl->halt = 1;
-        return;
-    }
-    double a = 9.e10, b = 0.0, c = 0.0;
-    for (int i = tid; i < nblk; i += 1024) {
+    a = 9.e10; b = 0.0; c = 0.0;
+    for (int i = lo + tid; i < hi; i += blockDim.x) {
         a = fmin(a, partials[3 * (size_t)i]);
         b = fmax(b, partials[3 * (size_t)i + 1]);
         c += partials[3 * (size_t)i + 2];
@@ -617,8 +615,13 @@ __device__ inline void clock_body(const fx_params &P, double *partials, int nblk
     a = wmin(a); b = wmax(b); c = wsum(c);
     if ((tid & 63) == 0) { sm[0][tid >> 6] = a; sm[1][tid >> 6] = b; sm[2][tid >> 6] = c; }
     __syncthreads();
-    if (tid != 0) return;
-    for (int w = 1; w < 16; w++) { a = fmin(a, sm[0][w]); b = fmax(b, sm[1][w]); c += sm[2][w]; }
+    if (tid == 0)
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) { a = fmin(a, sm[0][w]); b = fmax(b, sm[1][w]); c += sm[2][w]; }
+}
+
+// clock_finish: the step's clock from the folded (dt, hpall, weir) (one thread)
+__device__ inline void clock_finish(const fx_params &P, fx_ctl *ctl, double a, double b, double c)
+{
     // Kahan-compensated ledger of the weir-face exchange of the previous step
     {
         const double y = c * P.arbase - ctl->led_weir_c;
@@ -653,9 +656,54 @@ __device__ inline void clock_body(const fx_params &P, double *partials, int nblk
     ctl->pending_halt = (ctl->tim >= ctl->print_next);
 }
 
-__global__ __launch_bounds__(1024) void k32_clock(fx_params P, double *partials, int nblk, fx_ctl *ctl)
+// single block (batched members)
+__device__ inline void clock_body(const fx_params &P, double *partials, int nblk, fx_ctl *ctl)
 {
-    clock_body(P, partials, nblk, ctl);
+    if (ctl->halt) return;
+    if (ctl->tim > ctl->t_end) {                       // loop condition, main.cpp:271
+        if (threadIdx.x == 0) ctl->halt = 1;
+        return;
+    }
+    double a, b, c;
+    clock_fold(partials, 0, nblk, a, b, c);
+    if (threadIdx.x == 0) clock_finish(P, ctl, a, b, c);
+}
+
+// one domain: the partials are folded by several blocks in fixed chunks into
+// part2; the last block to finish (ticket) folds part2 in block order and
+// runs the clock, so the result does not depend on block timing
+__global__ __launch_bounds__(1024) void k32_clock(fx_params P, double *partials, int nblk, double *part2,
+                                                  uint32_t *ticket, fx_ctl *ctl)
+{
+    if (ctl->halt) return;
+    if (ctl->tim > ctl->t_end) {                       // loop condition, main.cpp:271
+        if (blockIdx.x == 0 && threadIdx.x == 0) ctl->halt = 1;
+        return;
+    }
+    const int chunk = (nblk + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int lo = min(nblk, (int)blockIdx.x * chunk), hi = min(nblk, lo + chunk);
+    double a, b, c;
+    clock_fold(partials, lo, hi, a, b, c);
+    __shared__ int last;
+    if (threadIdx.x == 0) {
+        part2[3 * blockIdx.x] = a;
+        part2[3 * blockIdx.x + 1] = b;
+        part2[3 * blockIdx.x + 2] = c;
+        __threadfence();
+        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last || threadIdx.x != 0) return;
+    __threadfence();
+    const volatile double *q = part2;
+    double A = 9.e10, B = 0.0, Cw = 0.0;
+    for (int k = 0; k < (int)gridDim.x; k++) {
+        A = fmin(A, q[3 * k]);
+        B = fmax(B, q[3 * k + 1]);
+        Cw += q[3 * k + 2];
+    }
+    *ticket = 0;
+    clock_finish(P, ctl, A, B, Cw);
 }
 
 // batched: one block per member
@@ -665,9 +713,13 @@ __global__ __launch_bounds__(1024) void k32b_clock(const fx_bslot *slots)
     clock_body(B.P, B.S.partials, B.npart, B.ctl);
 }
 
-__global__ void k32_flip(fx_ctl *ctl)
+// end of a step: the buffer flip and the print / end halt (k_step_end)
+__global__ void k32_flip_end(fx_ctl *ctl)
 {
-    if (!ctl->halt) ctl->cur ^= 1;
+    if (!ctl->halt) {
+        ctl->cur ^= 1;
+        ctl->halt = ctl->pending_halt;
+    }
 }
 
 // batched end of step: the buffer flip and the print/end halt of every
@@ -813,6 +865,8 @@ int fx32_init_state(fx_domain d)
     // room for the partials of the smallest column block fx_set_tiling allows
     const int64_t cap = (int64_t)g.x * ((P.ncols + FX32_CH_MIN - 1) / FX32_CH_MIN);
     if ((rc = dalloc32(d, &t.partials, 3 * cap))) return rc;
+    if ((rc = dalloc32(d, &t.part2, 3 * FX32_FOLD_MAX))) return rc;
+    if ((rc = dalloc32(d, &t.ticket, 1))) return rc;
     return FX_OK;
 }
 
@@ -1035,8 +1089,9 @@ int fx32_launch_step(fx_domain d)
 {
     fx_params P = fx_make_params(d);
     s32p S = make_s32p(d);
-    hipLaunchKernelGGL(k32_clock, dim3(1), dim3(1024), 0, d->stream, P, S.partials,
-                       (int)d->s32.npart, d->ctl);
+    const int nfold = (int)std::min<int64_t>(FX32_FOLD_MAX, std::max<int64_t>(1, (d->s32.npart + 1023) / 1024));
+    hipLaunchKernelGGL(k32_clock, dim3(nfold), dim3(1024), 0, d->stream, P, S.partials,
+                       (int)d->s32.npart, d->s32.part2, d->s32.ticket, d->ctl);
     FX_HIP(hipGetLastError());
     const int ade_on = (d->desc.ade_solver && d->desc.ade_mode == FX_ADE_FAST) ? 1 : 0;
     fx_prof_begin(d, FX_K_STEP32);
@@ -1047,8 +1102,8 @@ int fx32_launch_step(fx_domain d)
         int rc = fx32_launch_ade(d);
         if (rc) return rc;
     }
-    hipLaunchKernelGGL(k32_flip, dim3(1), dim3(1), 0, d->stream, d->ctl);
-    return fx_hip_check(hipGetLastError(), "k32_flip");
+    hipLaunchKernelGGL(k32_flip_end, dim3(1), dim3(1), 0, d->stream, d->ctl);
+    return fx_hip_check(hipGetLastError(), "k32_flip_end");
 }
 
 int fx32_launch_volume(fx_domain d, double *vol_out)

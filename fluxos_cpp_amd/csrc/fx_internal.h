@@ -76,6 +76,8 @@ struct fx_state32 {
     uint8_t *flags;            // FXF_* bits
     float *tw_acc, *tw_t0;     // wet-time tracer: closed runs, start of the open run
     double *partials;          // 3 per step-kernel block (dt, hpall, weir volume)
+    double *part2;             // 3 per clock block (second level of the fold)
+    uint32_t *ticket;          // last-block counter of the clock
     int64_t npart;
 };
 

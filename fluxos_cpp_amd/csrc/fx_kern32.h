@@ -35,6 +35,7 @@ struct a32p {
 #define FX32_CH_MIN 8         // smallest step-sweep block (sizes the CFL partials)
 #define FX32_AOWN 58          // rows owned per ADE-sweep strip
 #define FX32_ACH 64           // default columns per ADE-sweep block
+#define FX32_FOLD_MAX 256     // blocks of the clock's partial fold
 
 // One member of a batch, in device memory: everything its kernels need.
 struct fx_bslot {
