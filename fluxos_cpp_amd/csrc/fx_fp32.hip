@@ -619,8 +619,24 @@ __device__ inline void clock_fold(double *partials, int lo, int hi, double &a, d
         for (int w = 1; w < (int)(blockDim.x >> 6); w++) { a = fmin(a, sm[0][w]); b = fmax(b, sm[1][w]); c += sm[2][w]; }
 }
 
-// clock_finish: the step's clock from the folded (dt, hpall, weir) (one thread)
+__device__ inline void clock_advance(const fx_params &P, fx_ctl *ctl, double a, double b, double c);
+__device__ inline void clock_forcing(const fx_params &P, fx_ctl *ctl, int mrow, int irow);
+
+// clock_finish: the step's clock from the folded (dt, hpall, weir).  Every
+// thread of the block calls it (the forcing-table lookup is shared); thread 0
+// holds the folded values and writes the control block.
 __device__ inline void clock_finish(const fx_params &P, fx_ctl *ctl, double a, double b, double c)
+{
+    __shared__ double tim_new;
+    if (threadIdx.x == 0) clock_advance(P, ctl, a, b, c);
+    if (threadIdx.x == 0) tim_new = ctl->tim;
+    __syncthreads();
+    const int mrow = P.meteo ? fx_forcing_row_block(P.meteo, P.meteo_nrows, P.meteo_ncols, tim_new) : -1;
+    const int irow = P.inflow ? fx_forcing_row_block(P.inflow, P.inflow_nrows, P.inflow_ncols, tim_new) : -1;
+    if (threadIdx.x == 0) clock_forcing(P, ctl, mrow, irow);
+}
+
+__device__ inline void clock_advance(const fx_params &P, fx_ctl *ctl, double a, double b, double c)
 {
     // Kahan-compensated ledger of the weir-face exchange of the previous step
     {
@@ -646,7 +662,11 @@ __device__ inline void clock_finish(const fx_params &P, fx_ctl *ctl, double a, d
     ctl->k_dxy_dt = (float)(P.dxy / dt);
     ctl->s_wet_prev = ctl->s_wet;
     if (gate) ctl->s_wet = ctl->s_wet + dt / 3600;
-    fx_forcing_eval(P, ctl);
+}
+
+__device__ inline void clock_forcing(const fx_params &P, fx_ctl *ctl, int mrow, int irow)
+{
+    fx_forcing_eval(P, ctl, mrow, irow);
     if (P.meteo) {
         const double y = ctl->meteoi * ctl->melt_area * P.arbase - ctl->led_melt_c;
         const double t = ctl->led_melt + y;
@@ -666,7 +686,7 @@ __device__ inline void clock_body(const fx_params &P, double *partials, int nblk
     }
     double a, b, c;
     clock_fold(partials, 0, nblk, a, b, c);
-    if (threadIdx.x == 0) clock_finish(P, ctl, a, b, c);
+    clock_finish(P, ctl, a, b, c);
 }
 
 // one domain: the partials are folded by several blocks in fixed chunks into
@@ -693,16 +713,18 @@ __global__ __launch_bounds__(1024) void k32_clock(fx_params P, double *partials,
         last = atomicAdd(ticket, 1u) == gridDim.x - 1;
     }
     __syncthreads();
-    if (!last || threadIdx.x != 0) return;
-    __threadfence();
-    const volatile double *q = part2;
+    if (!last) return;
     double A = 9.e10, B = 0.0, Cw = 0.0;
-    for (int k = 0; k < (int)gridDim.x; k++) {
-        A = fmin(A, q[3 * k]);
-        B = fmax(B, q[3 * k + 1]);
-        Cw += q[3 * k + 2];
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const volatile double *q = part2;
+        for (int k = 0; k < (int)gridDim.x; k++) {
+            A = fmin(A, q[3 * k]);
+            B = fmax(B, q[3 * k + 1]);
+            Cw += q[3 * k + 2];
+        }
+        *ticket = 0;
     }
-    *ticket = 0;
     clock_finish(P, ctl, A, B, Cw);
 }
 

@@ -39,13 +39,31 @@ __device__ inline int fx_forcing_row(const float *tab, int nrows, int ncols, dou
     return -1;
 }
 
-// Evaluate both forcings for the clock held in ctl (one thread).
-__device__ inline void fx_forcing_eval(const fx_params &P, fx_ctl *ctl)
+// The same lookup by every thread of a block (all must call it): the smallest
+// row index whose time exceeds tim, -1 if none.
+__device__ inline int fx_forcing_row_block(const float *tab, int nrows, int ncols, double tim)
+{
+    __shared__ int best;
+    if (threadIdx.x == 0) best = 0x7fffffff;
+    __syncthreads();
+    int mine = 0x7fffffff;
+    for (int a = threadIdx.x; a < nrows; a += blockDim.x)
+        if ((double)tab[(size_t)a * ncols] > tim) { mine = a; break; }
+    if (mine != 0x7fffffff) atomicMin(&best, mine);
+    __syncthreads();
+    const int r = best;
+    __syncthreads();
+    return r == 0x7fffffff ? -1 : r;
+}
+
+// Evaluate both forcings for the clock held in ctl (one thread); mrow / irow
+// are the table rows if already looked up (-2 = look up here).
+__device__ inline void fx_forcing_eval(const fx_params &P, fx_ctl *ctl, int mrow = -2, int irow = -2)
 {
     double rate = 0.0;
     for (int k = 0; k < FX_MAX_CHEM; k++) { ctl->cmelt[k] = 0.0; ctl->cin[k] = 0.0; }
     if (P.meteo) {
-        int row = fx_forcing_row(P.meteo, P.meteo_nrows, P.meteo_ncols, ctl->tim);
+        int row = mrow != -2 ? mrow : fx_forcing_row(P.meteo, P.meteo_nrows, P.meteo_ncols, ctl->tim);
         if (row >= 0) {
             rate = (double)P.meteo[(size_t)row * P.meteo_ncols + 1];
             for (int k = 0; k < P.nchem && 2 + k < P.meteo_ncols; k++)
@@ -55,7 +73,7 @@ __device__ inline void fx_forcing_eval(const fx_params &P, fx_ctl *ctl)
     ctl->meteoi = rate / (1000. * 3600. * 24.) * ctl->dtfl;
     double q = 0.0;
     if (P.inflow) {
-        int row = fx_forcing_row(P.inflow, P.inflow_nrows, P.inflow_ncols, ctl->tim);
+        int row = irow != -2 ? irow : fx_forcing_row(P.inflow, P.inflow_nrows, P.inflow_ncols, ctl->tim);
         if (row >= 0) {
             q = (double)P.inflow[(size_t)row * P.inflow_ncols + 1];
             for (int k = 0; k < P.nchem && 2 + k < P.inflow_ncols; k++)
