@@ -28,10 +28,11 @@ ADE_OFF, ADE_EXACT, ADE_FAST = 0, 1, 2
 
 F_Z, F_H, F_QX, F_QY, F_US, F_LDRY, F_LDRY_PREV = 0, 1, 2, 3, 4, 5, 6
 F_FE1, F_FE2, F_FE3, F_FN1, F_FN2, F_FN3, F_TWET, F_H0, F_ZB = 7, 8, 9, 10, 11, 12, 13, 14, 15
+F_SOIL = 16
 F_CONC0 = 32
 FIELDS = {"z": F_Z, "h": F_H, "qx": F_QX, "qy": F_QY, "us": F_US, "ldry": F_LDRY,
           "ldry_prev": F_LDRY_PREV, "fe1": F_FE1, "fe2": F_FE2, "fe3": F_FE3, "fn1": F_FN1,
-          "fn2": F_FN2, "fn3": F_FN3, "twet": F_TWET, "h0": F_H0, "zb": F_ZB}
+          "fn2": F_FN2, "fn3": F_FN3, "twet": F_TWET, "h0": F_H0, "zb": F_ZB, "soil": F_SOIL}
 FLOAT_FIELDS = {F_LDRY, F_LDRY_PREV}
 
 EXPORTS = ("fx_domain_create", "fx_domain_destroy", "fx_domain_bytes", "fx_set_static",
@@ -41,7 +42,9 @@ EXPORTS = ("fx_domain_create", "fx_domain_destroy", "fx_domain_bytes", "fx_set_s
            "fx_synchronize", "fx_read_geo", "fx_initiation", "fx_last_error", "fx_version",
            "fx_profile", "fx_profile_read", "fx_batch_create", "fx_batch_destroy", "fx_batch_admit",
            "fx_batch_resume", "fx_batch_evict", "fx_batch_run", "fx_batch_poll", "fx_batch_info",
-           "fx_read_dem", "fx_read_forcing", "fx_write_results", "fx_set_tiling", "fx_get_tiling")
+           "fx_read_dem", "fx_read_forcing", "fx_write_results", "fx_set_tiling", "fx_get_tiling",
+           "fx_set_wintra", "fx_wintrasolver_calc", "fx_initiation_restart", "fx_read_results",
+           "fx_find_last_step")
 EV_PRINT, EV_DONE, EV_ERROR = 1, 2, 3
 K_CFL, K_MELT, K_FLUX, K_UPDATE, K_ADE_PASS1, K_ADE_SWEEP, K_STEP32, K_ADE32, K_CFL32 = range(9)
 KERNEL_NAMES = {K_CFL: "k64_cfl", K_MELT: "k64_melt", K_FLUX: "k64_flux", K_UPDATE: "k64_update",
@@ -77,7 +80,14 @@ class ResultsGrids(C.Structure):
     _fields_ = [("nrows", C.c_int32), ("ncols", C.c_int32), ("dxy", C.c_int32), ("pad", C.c_int32),
                 ("xllcorner", C.c_double), ("yllcorner", C.c_double), ("h_min_print", C.c_double)] + \
                [(k, C.POINTER(C.c_double)) for k in ("zb", "z", "h", "qx", "qy", "us", "conc0", "soil_mass",
-                                                      "fe1", "fn1", "twet")]
+                                                      "fe1", "fn1", "twet", "ux", "uy")]
+
+
+INIT_GRIDS = ("zb", "z", "h", "qx", "qy", "us", "ux", "uy", "conc0", "soil_mass", "twet")
+
+
+class InitGrids(C.Structure):
+    _fields_ = [(k, C.POINTER(C.c_double)) for k in INIT_GRIDS] + [("ldry", C.POINTER(C.c_float))]
 
 
 class StepInfo(C.Structure):
@@ -139,6 +149,12 @@ def load(path: Path | None = None) -> C.CDLL:
         "fx_read_forcing": ([C.c_char_p, C.c_int32, C.c_int32, _F, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
                             C.c_int),
         "fx_write_results": ([C.c_char_p, C.POINTER(ResultsGrids)], C.c_int),
+        "fx_set_wintra": ([vp, C.c_int32, C.c_double], C.c_int),
+        "fx_wintrasolver_calc": ([vp, C.c_int32], C.c_int),
+        "fx_initiation_restart": ([C.c_int32, C.c_int32, C.c_int32, C.c_double, _D, C.c_int64,
+                                   C.POINTER(InitGrids)], C.c_int),
+        "fx_read_results": ([C.c_char_p, C.c_int64, _D, C.POINTER(C.c_int64)], C.c_int),
+        "fx_find_last_step": ([C.c_char_p, C.POINTER(C.c_int32)], C.c_int),
         "fx_last_error": ([], C.c_char_p),
         "fx_version": ([], C.c_char_p),
     }
@@ -192,6 +208,39 @@ def initiation(zb: np.ndarray, ade_solver: bool = True):
     return zb, z, h, ldry, c0
 
 
+def initiation_restart(zb: np.ndarray, table: np.ndarray, ade_solver: bool = True,
+                       soil_conc_bckgrd: float = 0.0) -> dict:
+    """initiate.cpp:27-231 with RESTART true: zb from read_geo and a Results
+    file's data rows; returns every initial grid by name (zb included)."""
+    lib = load()
+    nr, nc = zb.shape[0] - 2, zb.shape[1] - 2
+    out = {k: np.zeros(zb.shape, order="F") for k in INIT_GRIDS}
+    out["zb"][...] = zb
+    out["ldry"] = np.zeros(zb.shape, dtype=np.float32, order="F")
+    g = InitGrids(*[_dp(out[k]) for k in INIT_GRIDS], _fp(out["ldry"]))
+    t = np.ascontiguousarray(table, dtype=np.float64).reshape(-1, 16)
+    _check(lib.fx_initiation_restart(nr, nc, 1 if ade_solver else 0, float(soil_conc_bckgrd), _dp(t),
+                                     t.shape[0], C.byref(g)))
+    return out
+
+
+def read_results_file(path) -> np.ndarray:
+    """The data rows of a Results/<t>.txt file (n x 16 float64)."""
+    lib = load()
+    n = C.c_int64()
+    _check(lib.fx_read_results(str(path).encode(), 0, None, C.byref(n)))
+    tab = np.zeros((n.value, 16))
+    _check(lib.fx_read_results(str(path).encode(), n.value, _dp(tab), C.byref(n)))
+    return tab
+
+
+def find_last_step(folder) -> int:
+    """findLastStep (common.cpp:69-106) on a Results folder."""
+    t = C.c_int32()
+    _check(load().fx_find_last_step(str(folder).encode(), C.byref(t)))
+    return t.value
+
+
 @dataclass
 class InitialState:
     zb: np.ndarray
@@ -226,10 +275,11 @@ def read_forcing_file(path, ncols: int | None = None) -> np.ndarray:
 
 
 def write_results_file(path, dxy: int, xll: float, yll: float, h_min_print: float, zb, z, h, qx=None,
-                       qy=None, us=None, conc0=None, soil_mass=None, fe1=None, fn1=None, twet=None) -> None:
+                       qy=None, us=None, conc0=None, soil_mass=None, fe1=None, fn1=None, twet=None,
+                       ux=None, uy=None) -> None:
     """Results/<t>.txt from padded Fortran-ordered float64 grids."""
     keep = [None if a is None else np.asfortranarray(a, dtype=np.float64)
-            for a in (zb, z, h, qx, qy, us, conc0, soil_mass, fe1, fn1, twet)]
+            for a in (zb, z, h, qx, qy, us, conc0, soil_mass, fe1, fn1, twet, ux, uy)]
     ptr = [None if a is None else _dp(a) for a in keep]
     nr, nc = zb.shape[0] - 2, zb.shape[1] - 2
     g = ResultsGrids(nr, nc, int(dxy), 0, float(xll), float(yll), float(h_min_print), *ptr)
@@ -261,6 +311,7 @@ class HipDomain:
         self.shape = (self.nrows + 2, self.ncols + 2)
         self.nchem = int(nchem)
         self.precision = precision
+        self.wintra = False
         d = DomainDesc()
         d.nrows, d.ncols, d.dxy, d.nchem = self.nrows, self.ncols, int(dxy), self.nchem
         d.hdry, d.nodata = float(hdry), float(nodata)
@@ -417,15 +468,25 @@ class HipDomain:
     SNAPSHOT_FIELDS = ("z", "h", "qx", "qy", "fe1", "fe2", "fe3", "fn1", "fn2", "fn3", "twet", "ldry",
                        "ldry_prev", "h0")
 
+    def set_wintra(self, on: bool, soil_release_rate: float) -> None:
+        """EXTERNAL_MODULES.WINTRA.STATUS and SOIL_RELEASE_RATE [1/h]."""
+        _check(self.lib.fx_set_wintra(self.h, 1 if on else 0, float(soil_release_rate)))
+        self.wintra = bool(on)
+
+    def wintrasolver_calc(self, ichem: int = 0) -> None:
+        _check(self.lib.fx_wintrasolver_calc(self.h, int(ichem)))
+
     def snapshot(self, solutes: bool = True) -> dict:
         """Every grid a run restarts from (reference layout) and the clock."""
         out = {k: self.get(k) for k in self.SNAPSHOT_FIELDS}
+        if getattr(self, "wintra", False):
+            out["soil"] = self.get("soil")
         out["conc"] = [self.get_conc(k) for k in range(self.nchem)] if solutes else []
         out["clock"] = self.clock()
         return out
 
     def restore(self, snap: dict) -> None:
-        for k in self.SNAPSHOT_FIELDS:
+        for k in self.SNAPSHOT_FIELDS + ("soil",):
             if k in snap:
                 self.set(k, snap[k])
         for k, c in enumerate(snap.get("conc", [])):

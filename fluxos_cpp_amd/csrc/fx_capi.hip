@@ -94,6 +94,8 @@ fx_params fx_make_params(fx_domain d)
     P.rates_nonneg = d->rates_nonneg ? 1 : 0;
     P.ch = d->ch;
     P.ach = d->ach;
+    P.wintra = d->wintra ? 1 : 0;
+    P.soil_rate = d->soil_rate;
     P.dxy = (double)D.dxy;
     P.arbase = (double)((int64_t)D.dxy * (int64_t)D.dxy);
     P.dxy_u = (uint32_t)D.dxy;
@@ -165,8 +167,8 @@ int check_desc(const fx_domain_desc *D)
     if (D->ade_mode < FX_ADE_OFF || D->ade_mode > FX_ADE_FAST) return fx_fail(FX_ERR_ARG, "unknown ade_mode");
     if (D->precision == FX_FP64_PARITY && D->ade_solver && D->ade_mode == FX_ADE_FAST)
         return fx_fail(FX_ERR_UNSUPPORTED, "FP64 parity domains run the exact ADE only");
-    if (D->precision == FX_FP32_MIXED && D->ade_solver && D->ade_mode == FX_ADE_EXACT)
-        return fx_fail(FX_ERR_UNSUPPORTED, "FP32-mixed domains run the fast ADE only");
+    if (D->precision == FX_FP32_MIXED && D->ade_solver && D->ade_mode != FX_ADE_FAST)
+        return fx_fail(FX_ERR_UNSUPPORTED, "FP32-mixed domains with the ADE solver run the fast ADE");
     return FX_OK;
 }
 
@@ -197,6 +199,20 @@ bool ring_is_weir(const fx_domain_desc &D, int mrows, const float *weir)
     for (int irow = 1; irow <= D.nrows; irow++)
         if (weir[irow + (size_t)1 * mrows] != 1.0f || weir[irow + (size_t)D.ncols * mrows] != 1.0f) return false;
     return true;
+}
+
+// the soil_mass grid, allocated (zeroed) on first need
+int soil_alloc(fx_domain d)
+{
+    if (d->soil) return FX_OK;
+    if (hipMalloc((void **)&d->soil, (size_t)d->ncell * 8) != hipSuccess) {
+        d->soil = nullptr;
+        return fx_fail(FX_ERR_NOMEM, "hipMalloc of the soil_mass grid failed");
+    }
+    d->bytes += d->ncell * 8;
+    FX_HIP(hipMemsetAsync(d->soil, 0, (size_t)d->ncell * 8, d->stream));
+    FX_HIP(hipStreamSynchronize(d->stream));
+    return FX_OK;
 }
 
 // double field pointer of an FP64 domain
@@ -241,6 +257,7 @@ int launch_step64(fx_domain d)
     if (d->desc.ade_solver && d->desc.ade_mode != FX_ADE_OFF)
         for (int k = 0; k < d->desc.nchem; k++)
             if ((rc = fx64_launch_ade_exact(d, G, k, 0))) return rc;
+    if (d->wintra && (rc = fx64_launch_wintra(d, G, 0, d->desc.nchem))) return rc;
     return fx_launch_step_end(d);
 }
 
@@ -341,6 +358,7 @@ int fx_domain_destroy(fx_domain d)
     for (int k = 0; k < FX_MAX_CHEM; k++) { dfree(t.mass[0][k]); dfree(t.mass[1][k]); }
     for (auto &r : d->prof.pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     for (auto e : d->prof.pool) (void)hipEventDestroy(e);
+    dfree(d->soil);
     dfree(d->ctl);
     dfree(d->partials);
     dfree(d->ade_progress);
@@ -402,6 +420,12 @@ int fx_set_field(fx_domain d, int32_t field, const void *host)
         return FX_OK;
     }
     if (field == FX_F_ZB) return fx_fail(FX_ERR_ARG, "zb is set with fx_set_static");
+    if (field == FX_F_SOIL) {
+        if ((rc = soil_alloc(d))) return rc;
+        FX_HIP(hipMemcpyAsync(d->soil, host, mm * 8, hipMemcpyHostToDevice, d->stream));
+        FX_HIP(hipStreamSynchronize(d->stream));
+        return FX_OK;
+    }
     if (d->desc.precision == FX_FP32_MIXED) return fx32_set_field(d, field, host);
     if (double *p = f64_ptr(d, field)) {
         FX_HIP(hipMemcpyAsync(p, host, mm * 8, hipMemcpyHostToDevice, d->stream));
@@ -432,6 +456,12 @@ int fx_get_field(fx_domain d, int32_t field, void *host)
         memset(host, 0, mm * 8);
         return FX_OK;
     }
+    if (field == FX_F_SOIL) {
+        if (!d->soil) { memset(host, 0, mm * 8); return FX_OK; }
+        FX_HIP(hipMemcpyAsync(host, d->soil, mm * 8, hipMemcpyDeviceToHost, d->stream));
+        FX_HIP(hipStreamSynchronize(d->stream));
+        return FX_OK;
+    }
     if (d->desc.precision == FX_FP32_MIXED) return fx32_get_field(d, field, host);
     if (double *p = f64_ptr(d, field)) {
         FX_HIP(hipMemcpyAsync(host, p, mm * 8, hipMemcpyDeviceToHost, d->stream));
@@ -447,6 +477,36 @@ int fx_get_field(fx_domain d, int32_t field, void *host)
         return FX_OK;
     }
     return fx_fail(FX_ERR_ARG, "unknown field");
+}
+
+int fx_set_wintra(fx_domain d, int32_t status, double soil_release_rate)
+{
+    int rc = enter(d);
+    if (rc) return rc;
+    if (d->in_batch) return fx_fail(FX_ERR_STATE, "domain is a batch member");
+    if (!std::isfinite(soil_release_rate)) return fx_fail(FX_ERR_ARG, "soil_release_rate must be finite");
+    if (status) {
+        if (!d->desc.ade_solver)
+            return fx_fail(FX_ERR_STATE, "WINTRA needs the ADE solver (the reference turns it off without)");
+        if ((rc = soil_alloc(d))) return rc;
+    }
+    d->wintra = status != 0;
+    d->soil_rate = soil_release_rate;
+    if (d->graph_ok) { (void)hipGraphExecDestroy(d->graph); d->graph_ok = false; }
+    return FX_OK;
+}
+
+int fx_wintrasolver_calc(fx_domain d, int32_t ichem)
+{
+    int rc = enter(d);
+    if (rc) return rc;
+    if (d->desc.precision != FX_FP64_PARITY)
+        return fx_fail(FX_ERR_UNSUPPORTED, "granular calls need an FP64 parity domain");
+    if (!d->wintra) return fx_fail(FX_ERR_STATE, "WINTRA is off for this domain (fx_set_wintra)");
+    if (ichem < 0 || ichem >= d->desc.nchem) return fx_fail(FX_ERR_ARG, "ichem out of range");
+    if ((rc = fx64_launch_wintra(d, 0, ichem, 1))) return rc;
+    FX_HIP(hipStreamSynchronize(d->stream));
+    return FX_OK;
 }
 
 int fx_set_clock(fx_domain d, double tim, double dtfl, int32_t it)

@@ -735,6 +735,30 @@ __global__ __launch_bounds__(1024) void k32b_clock(const fx_bslot *slots)
     clock_body(B.P, B.S.partials, B.npart, B.ctl);
 }
 
+// WINTRAsolver_calc.cpp:26-57 on the step's new state (before the flip):
+// a wet cell whose bed is not NODATA releases soil_rate/3600 of its soil mass
+// per second into each solute in turn; the solute is held as mass per unit
+// area, so c += dm / (h dxy^2) becomes m += dm / dxy^2
+__global__ __launch_bounds__(256) void k32_wintra(fx_params P, a32p A, double *soil, fx_ctl *ctl)
+{
+    if (ctl->halt || !ctl->gate) return;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= A.mm) return;
+    const uint8_t f = A.flags[i];
+    if (!(f & FXF_INNER) || (f & FXF_NODATA)) return;
+    const int nxt = ctl->cur ^ 1;
+    const double hp = A.d[nxt][i];
+    if (!(hp > P.hdry)) return;
+    const double dtl = ctl->dtfl;
+    double m = soil[i];
+    for (int k = 0; k < P.nchem; k++) {
+        const double deltam = m * P.soil_rate / 3600 * dtl;
+        m = m - deltam;
+        A.mass[nxt][k][i] += (float)(deltam / P.arbase);
+    }
+    soil[i] = m;
+}
+
 // end of a step: the buffer flip and the print / end halt (k_step_end)
 __global__ void k32_flip_end(fx_ctl *ctl)
 {
@@ -1107,6 +1131,15 @@ int fx32_launch_cfl_initial(fx_domain d)
     return fx_hip_check(hipGetLastError(), "k32_cfl");
 }
 
+int fx32_launch_wintra(fx_domain d)
+{
+    fx_params P = fx_make_params(d);
+    a32p A = make_a32p(d);
+    const unsigned nb = (unsigned)((d->ncell + 255) / 256);
+    hipLaunchKernelGGL(k32_wintra, dim3(nb), dim3(256), 0, d->stream, P, A, d->soil, d->ctl);
+    return fx_hip_check(hipGetLastError(), "k32_wintra");
+}
+
 int fx32_launch_step(fx_domain d)
 {
     fx_params P = fx_make_params(d);
@@ -1122,6 +1155,10 @@ int fx32_launch_step(fx_domain d)
     FX_HIP(hipGetLastError());
     if (ade_on) {
         int rc = fx32_launch_ade(d);
+        if (rc) return rc;
+    }
+    if (d->wintra) {
+        int rc = fx32_launch_wintra(d);
         if (rc) return rc;
     }
     hipLaunchKernelGGL(k32_flip_end, dim3(1), dim3(1), 0, d->stream, d->ctl);

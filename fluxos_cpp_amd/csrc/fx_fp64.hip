@@ -489,6 +489,29 @@ __global__ __launch_bounds__(256) void k64_update(fx_params P, fx_state64 s, fx_
     }
 }
 
+// WINTRAsolver_calc.cpp:26-57: a wet cell whose bed is not NODATA releases
+// soil_rate/3600 of its soil mass per second of the step into the solute,
+// once per solute in solute order (main.cpp:400-403)
+__global__ __launch_bounds__(256) void k64_wintra(fx_params P, fx_state64 s, double *soil, fx_ctl *ctl,
+                                                  unsigned mode, int k_first, int k_count)
+{
+    if ((mode & FXM_GATED) && (ctl->halt || !ctl->gate)) return;
+    int irow, icol;
+    if (!fx_interior(P, irow, icol)) return;
+    const size_t p = (size_t)irow + (size_t)icol * P.ld;
+    const double hp = s.h[p];
+    const double zbp = s.zb[p];
+    if (!(hp > P.hdry && zbp != P.nodata)) return;
+    const double dtl = ctl->dtfl;
+    double m = soil[p];
+    for (int k = k_first; k < k_first + k_count; k++) {
+        const double deltam = m * P.soil_rate / 3600 * dtl;
+        m = m - deltam;
+        s.conc[k][p] = s.conc[k][p] + deltam / (hp * P.arbase);
+    }
+    soil[p] = m;
+}
+
 __global__ void k_forcing(fx_params P, fx_ctl *ctl) { fx_forcing_eval(P, ctl); }
 
 __global__ void k_step_end(fx_ctl *ctl)
@@ -547,6 +570,14 @@ int fx64_launch_flux(fx_domain d, unsigned mode)
     hipLaunchKernelGGL(k64_flux, grid_of(P), dim3(BX, BY), 0, d->stream, P, d->s64, d->ctl, mode);
     fx_prof_end(d, FX_K_FLUX);
     return fx_hip_check(hipGetLastError(), "k64_flux");
+}
+
+int fx64_launch_wintra(fx_domain d, unsigned mode, int k_first, int k_count)
+{
+    fx_params P = fx_make_params(d);
+    hipLaunchKernelGGL(k64_wintra, grid_of(P), dim3(BX, BY), 0, d->stream, P, d->s64, d->soil, d->ctl, mode,
+                       k_first, k_count);
+    return fx_hip_check(hipGetLastError(), "k64_wintra");
 }
 
 int fx64_launch_update(fx_domain d, unsigned mode)

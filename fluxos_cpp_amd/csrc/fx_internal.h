@@ -123,6 +123,9 @@ struct fx_domain_s {
     bool graph_ok;
     int32_t ch, ach;           // FP32 sweep column-block lengths (fx_set_tiling)
     bool in_batch;             // member of a batch: tiling is fixed
+    bool wintra;               // WINTRA soil release on (fx_set_wintra)
+    double soil_rate;          // SOIL_RELEASE_RATE [1/h]
+    double *soil;              // soil_mass grid (FP64, both precisions); NULL until needed
     int64_t bytes;
     fx_prof prof;
 };
@@ -146,6 +149,8 @@ int fx64_launch_ade_exact(fx_domain d, unsigned mode, int ichem, int it_override
 int fx_launch_forcing(fx_domain d);
 int fx_launch_step_end(fx_domain d);
 int fx64_launch_volume(fx_domain d, double *vol_out);
+int fx64_launch_wintra(fx_domain d, unsigned mode, int k_first, int k_count);
+int fx32_launch_wintra(fx_domain d);
 
 int fx32_init_state(fx_domain d);
 int fx32_launch_step(fx_domain d);

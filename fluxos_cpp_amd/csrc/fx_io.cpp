@@ -1,12 +1,15 @@
 // fx_io.cpp -- the reference's file formats at the boundary of the step:
 // the ESRI-ASCII DEM (read_functions.cpp:112-164, common.cpp:110-161), the
-// forcing CSV files (read_functions.cpp:237-326) and the Results/<t>.txt
-// writer (write_results.cpp:30-102).  Host code only; no GPU needed.
+// forcing CSV files (read_functions.cpp:237-326), the Results/<t>.txt
+// writer (write_results.cpp:30-102) and its reader for a restart
+// (initiate.cpp:109-156, common.cpp:69-106).  Host code only; no GPU needed.
 #include <algorithm>
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <dirent.h>
 #include <fstream>
 #include <sstream>
 #include <string>
@@ -165,8 +168,8 @@ int fx_write_results(const char *path, const fx_results_grids *g)
             v[3] = (double)irow * g->dxy + g->yllcorner;
             v[4] = g->z[p];
             v[5] = g->z[p] - g->zb[p];
-            v[6] = 0.0;                                  // ux, uy: the reference never updates them
-            v[7] = 0.0;
+            v[6] = val(g->ux, p);                        // ux, uy: never updated by the solver,
+            v[7] = val(g->uy, p);                        // only carried over from a restart file
             v[8] = val(g->qx, p) * g->dxy;
             v[9] = val(g->qy, p) * g->dxy;
             v[10] = val(g->us, p);
@@ -186,6 +189,55 @@ int fx_write_results(const char *path, const fx_results_grids *g)
     }
     const bool ok = std::fclose(f) == 0;
     return ok ? FX_OK : fx_fail(FX_ERR_IO, std::string("cannot write ") + path);
+}
+
+int fx_read_results(const char *path, int64_t max_rows, double *table, int64_t *nrows)
+{
+    if (!path || !nrows) return fx_fail(FX_ERR_ARG, "path or nrows is NULL");
+    std::ifstream f(path);
+    if (!f.is_open()) return fx_fail(FX_ERR_IO, std::string("cannot open Results file ") + path);
+    std::string s;
+    int64_t n = 0;
+    bool first = true;
+    std::vector<double> row;
+    while (std::getline(f, s)) {
+        if (first) { first = false; continue; }         // the header line
+        if (s.find_first_not_of(" \t\r\n") == std::string::npos) continue;
+        row.assign(16, 0.0);
+        std::stringstream ss(s);
+        std::string cell;
+        int j = 0;
+        while (j < 16 && std::getline(ss, cell, ',')) {
+            char *end = nullptr;
+            const double x = std::strtod(cell.c_str(), &end);
+            row[j++] = end == cell.c_str() ? 0.0 : x;
+        }
+        if (table) {
+            if (n >= max_rows) return fx_fail(FX_ERR_ARG, "Results file has more rows than the buffer");
+            std::copy(row.begin(), row.end(), table + n * 16);
+        }
+        n++;
+    }
+    if (first) return fx_fail(FX_ERR_IO, std::string("Results file is empty: ") + path);
+    *nrows = n;
+    return FX_OK;
+}
+
+int fx_find_last_step(const char *folder, int32_t *timstart)
+{
+    if (!folder || !timstart) return fx_fail(FX_ERR_ARG, "folder or timstart is NULL");
+    int t = 0;
+    if (DIR *dir = opendir(folder)) {
+        while (dirent *e = readdir(dir)) {
+            if (!std::strcmp(e->d_name, ".") || !std::strcmp(e->d_name, "..")) continue;
+            char head[7] = {0};
+            std::strncpy(head, e->d_name, 6);
+            t = std::max(t, std::atoi(head));
+        }
+        closedir(dir);
+    }
+    *timstart = t;
+    return FX_OK;
 }
 
 } // extern "C"

@@ -12,6 +12,8 @@ struct fx_params {
     int32_t inflow_nrows, inflow_ncols;
     int32_t rates_nonneg;
     int32_t ch, ach;          // column-block lengths of the FP32 step and ADE sweeps
+    int32_t wintra;           // WINTRA module on (soil release after the ADE)
+    double soil_rate;         // SOIL_RELEASE_RATE [1/h]
     int64_t inflow_p;         // linear index of the inflow cell, -1 = none
     double dxy;               // CELLSIZE as double
     double arbase;            // dxy*dxy as double

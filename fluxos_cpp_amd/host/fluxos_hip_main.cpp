@@ -6,6 +6,8 @@
 //
 // Default precision is FX_FP64_PARITY with the exact ADE, which reproduces the
 // reference bit for bit; --fp32 selects the FP32-mixed path with the fast ADE.
+// RESTART (the newest Results file as initial conditions) and the WINTRA
+// soil release are honoured as the reference does them; OPENWQ is refused.
 // Steps between two print times run device-resident (fx_advance); the state
 // is copied back only to write a Results file.
 #include <algorithm>
@@ -59,12 +61,13 @@ int die(Log &log, const std::string &msg, int rc = 1)
     return rc;
 }
 
-// the state grids a Results file needs, fetched from the device
+// the state grids a Results file needs; soil_mass comes from the device only
+// while WINTRA changes it, ux / uy never change (initiate.cpp, write_results.cpp:60-61)
 struct Grids {
-    std::vector<double> zb, z, h, qx, qy, us, conc0, fe1, fn1, twet;
+    std::vector<double> zb, z, h, qx, qy, us, conc0, fe1, fn1, twet, soil, ux, uy;
 };
 
-int fetch(fx_domain dom, int64_t mm, Grids &g)
+int fetch(fx_domain dom, int64_t mm, bool wintra, Grids &g)
 {
     auto get = [&](int field, std::vector<double> &v) {
         v.resize(mm);
@@ -75,6 +78,7 @@ int fetch(fx_domain dom, int64_t mm, Grids &g)
         (rc = get(FX_F_QY, g.qy)) || (rc = get(FX_F_US, g.us)) || (rc = get(FX_F_CONC0, g.conc0)) ||
         (rc = get(FX_F_FE1, g.fe1)) || (rc = get(FX_F_FN1, g.fn1)) || (rc = get(FX_F_TWET, g.twet)))
         return rc;
+    if (wintra && (rc = get(FX_F_SOIL, g.soil))) return rc;
     return FX_OK;
 }
 
@@ -118,20 +122,29 @@ int main(int argc, char **argv)
             return die(log, e.what());
         }
     }
-    const std::string dem_file = str(m.find("DEM_FILE"));
+    // compulsory keys (read_functions.cpp:39-65): a missing one is the
+    // reference's json type_error
     const fxjson::Value *out = m.find("OUTPUT");
-    if (dem_file.empty() || !out || !m.find("ROUGNESS_HEIGHT"))
-        return die(log, "PROBLEM loading of master input file: " + modset_path);
+    for (const fxjson::Value *v :
+         {m.find("DEM_FILE"), m.find("SIM_DATETIME_START"), m.find("RESTART"), out,
+          out ? out->find("OUTPUT_FOLDER") : nullptr, out ? out->find("PRINT_STEP") : nullptr,
+          out ? out->find("H_MIN_TO_PRINT") : nullptr, m.find("ROUGNESS_HEIGHT"), m.find("SOIL_RELEASE_RATE"),
+          m.find("SOIL_CONC_BACKGROUND"), m.at({"EXTERNAL_MODULES", "ADE_TRANSPORT", "STATUS"}),
+          m.at({"EXTERNAL_MODULES", "OPENWQ", "STATUS"}), m.at({"EXTERNAL_MODULES", "WINTRA", "STATUS"})})
+        if (!v) return die(log, "PROBLEM loading of master input file: " + modset_path);
+    const std::string dem_file = str(m.find("DEM_FILE"));
     const std::string out_dir = str(out->find("OUTPUT_FOLDER"));
     const unsigned long print_step = (unsigned long)num(out->find("PRINT_STEP"), 3600.0);
     const double h_min_print = num(out->find("H_MIN_TO_PRINT"), 0.0);
     const double ks_input = num(m.find("ROUGNESS_HEIGHT"), 0.0);
+    const double soil_release_rate = num(m.find("SOIL_RELEASE_RATE"), 0.0);
+    const double soil_conc_bckgrd = num(m.find("SOIL_CONC_BACKGROUND"), 0.0);
+    const bool restart = flag(m.find("RESTART"));
     const bool ade = flag(m.at({"EXTERNAL_MODULES", "ADE_TRANSPORT", "STATUS"}));
     const bool openwq = flag(m.at({"EXTERNAL_MODULES", "OPENWQ", "STATUS"}));
-    const bool wintra = flag(m.at({"EXTERNAL_MODULES", "WINTRA", "STATUS"}));
-    if (flag(m.find("RESTART"))) return die(log, "RESTART from a Results file is not supported by this driver");
-    if (openwq) return die(log, "OPENWQ coupling is not available in this build");
-    if (wintra) return die(log, "WINTRA is not available in this build");
+    // without the ADE solver the reference switches WINTRA (and OpenWQ) off (main.cpp:207-213)
+    const bool wintra = ade && flag(m.at({"EXTERNAL_MODULES", "WINTRA", "STATUS"}));
+    if (openwq && ade) return die(log, "OPENWQ coupling is not available in this build");
     int nchem = 1;
     double dcoef[FX_MAX_CHEM] = {0};
     if (ade) {
@@ -209,10 +222,47 @@ int main(int argc, char **argv)
     const int ntim = std::max(ntim_meteo, ntim_inflow);
     log("Simulation time (days) = " + std::to_string(ntim) + " (= " + std::to_string(ntim) + " sec)");
 
-    // ---- initiation, cold start (initiate.cpp:27-231) -------------------------
-    if (fx_initiation(nr, nc, ade ? 1 : 0, zb.data(), z.data(), h.data(), ldry.data(), conc0.data()))
-        return die(log, fx_last_error());
-    const unsigned timstart = 0;
+    log("Soil release rate (1/hour) = " + std::to_string(soil_release_rate));
+    log("Soil initial background mass available for release to runoff (g) = " + std::to_string(soil_conc_bckgrd));
+    if (ade) log(" > ADE_solver activated");
+    if (wintra) log(" > WINTRA activated");
+
+    // the output folder is created first (initiate.cpp:40)
+    if (!out_dir.empty()) {
+        std::error_code ec;
+        std::filesystem::create_directories(out_dir, ec);
+    }
+
+    // ---- initiation (initiate.cpp:27-231) -----------------------------------------
+    std::vector<double> qx0(mm, 0.0), qy0(mm, 0.0), us0(mm, 0.0), twet0(mm, 0.0), soil(mm, 0.0);
+    std::vector<double> ux(mm, 0.0), uy(mm, 0.0);
+    unsigned timstart = 0;
+    bool loaded = false;
+    if (restart) {
+        // the newest Results file (initiate.cpp:116-120, common.cpp:69-106)
+        int32_t ts = 0;
+        if (fx_find_last_step((out_dir + "/").c_str(), &ts)) return die(log, fx_last_error());
+        timstart = (unsigned)ts;
+        const std::string init_file = out_dir + "/" + "/" + std::to_string(ts) + ".txt";
+        int64_t n = 0;
+        if (fx_read_results(init_file.c_str(), 0, nullptr, &n) == FX_OK) {
+            std::vector<double> table((size_t)std::max<int64_t>(1, n) * 16);
+            if (fx_read_results(init_file.c_str(), n, table.data(), &n)) return die(log, fx_last_error());
+            fx_init_grids ig{zb.data(), z.data(), h.data(), qx0.data(), qy0.data(), us0.data(), ux.data(),
+                             uy.data(), conc0.data(), soil.data(), twet0.data(), ldry.data()};
+            if (fx_initiation_restart(nr, nc, ade ? 1 : 0, soil_conc_bckgrd, table.data(), n, &ig))
+                return die(log, fx_last_error());
+            loaded = true;
+            log("RESTART option TRUE: Successful loading of initial conditions file: " + init_file);
+        } else {
+            log("RESTART option TRUE, but NO INITIAL CONDITIONS FOUND: All variables set to zero");
+        }
+    }
+    if (!loaded) {
+        if (fx_initiation(nr, nc, ade ? 1 : 0, zb.data(), z.data(), h.data(), ldry.data(), conc0.data()))
+            return die(log, fx_last_error());
+        if (!restart) log("RESTART option FALSE: All variables set to zero");
+    }
 
     // ---- device domain ----------------------------------------------------------
     fx_domain_desc desc{};
@@ -242,28 +292,32 @@ int main(int argc, char **argv)
     if (fx_set_field(dom, FX_F_Z, z.data()) || fx_set_field(dom, FX_F_H, h.data()) ||
         fx_set_field(dom, FX_F_LDRY, ldry.data()) || fx_set_field(dom, FX_F_CONC0, conc0.data()))
         return fail("fx_set_field");
+    if (loaded && (fx_set_field(dom, FX_F_QX, qx0.data()) || fx_set_field(dom, FX_F_QY, qy0.data()) ||
+                   fx_set_field(dom, FX_F_US, us0.data()) || fx_set_field(dom, FX_F_TWET, twet0.data())))
+        return fail("fx_set_field");
+    if (wintra && (fx_set_wintra(dom, 1, soil_release_rate) || fx_set_field(dom, FX_F_SOIL, soil.data())))
+        return fail("fx_set_wintra");
     if (!meteo.empty() && fx_set_meteo(dom, mrows + 1, tcols, meteo.data(), nullptr)) return fail("fx_set_meteo");
     if (!inflow_tab.empty() && fx_set_inflow(dom, irows, tcols, inflow_tab.data(), in_irow, in_icol))
         return fail("fx_set_inflow");
-    if (fx_set_clock(dom, (double)timstart, 0.0, 0)) return fail("fx_set_clock");
+    if (fx_set_clock(dom, 0.0, 0.0, 0)) return fail("fx_set_clock");
 
-    // the reference expects OUTPUT_FOLDER to exist; create it if it does not
-    if (!out_dir.empty()) {
-        std::error_code ec;
-        std::filesystem::create_directories(out_dir, ec);
-    }
-
+    // ds.tim is not moved to timstart on a restart (main.cpp:255-261): the
+    // clock starts at 0 and the first print is at timstart + PRINT_STEP
     // ---- time loop (main.cpp:271-470) ---------------------------------------------
     unsigned long print_next = timstart + print_step;
     Grids g;
     g.zb.resize(mm);
+    g.soil = soil;
+    g.ux = ux;
+    g.uy = uy;
     if (fx_get_field(dom, FX_F_ZB, g.zb.data())) return fail("fx_get_field");
     auto t_last = std::chrono::steady_clock::now();
     for (;;) {
         fx_step_info info{};
         if (fx_advance(dom, (double)ntim, (double)print_next, 1 << 30, &info)) return fail("fx_advance");
         if (info.status & 1) {
-            if (fetch(dom, mm, g)) return fail("fx_get_field");
+            if (fetch(dom, mm, wintra, g)) return fail("fx_get_field");
             fx_results_grids rg{};
             rg.nrows = nr;
             rg.ncols = nc;
@@ -274,6 +328,7 @@ int main(int argc, char **argv)
             rg.zb = g.zb.data(); rg.z = g.z.data(); rg.h = g.h.data();
             rg.qx = g.qx.data(); rg.qy = g.qy.data(); rg.us = g.us.data();
             rg.conc0 = g.conc0.data(); rg.fe1 = g.fe1.data(); rg.fn1 = g.fn1.data(); rg.twet = g.twet.data();
+            rg.soil_mass = g.soil.data(); rg.ux = g.ux.data(); rg.uy = g.uy.data();
             const std::string path = out_dir + "/" + std::to_string((long)std::llround((double)print_next)) + ".txt";
             if (fx_write_results(path.c_str(), &rg)) return fail("Problem when saving the results");
             const auto now = std::chrono::steady_clock::now();

@@ -81,6 +81,7 @@ typedef struct fx_domain_desc {
 #define FX_F_TWET      13   /* double  twetimetracer                      */
 #define FX_F_H0        14   /* double  h0                                  */
 #define FX_F_ZB        15   /* double  zb (read only; set by fx_set_static)*/
+#define FX_F_SOIL      16   /* double  soil_mass (WINTRA; GlobVar.h:68)    */
 #define FX_F_CONC0     32   /* double  conc_SW[k] is FX_F_CONC0 + k        */
 
 /* Per-call step summary of fx_advance. */
@@ -141,6 +142,19 @@ int fx_add_inflow(fx_domain dom);
 int fx_hydrodynamics_calc(fx_domain dom);
 /* adesolver_calc, ADEsolver_calc.cpp:27-291 (ADEsolver_calc.h:25). */
 int fx_adesolver_calc(fx_domain dom, int32_t it, int32_t ichem);
+/* wintrasolver_calc, WINTRAsolver_calc.cpp:26-57 (WINTRAsolver_calc.h:25):
+ * release of soil_mass into solute ichem in every wet cell whose bed is not
+ * NODATA.  Needs fx_set_wintra(dom, 1, ...). */
+int fx_wintrasolver_calc(fx_domain dom, int32_t ichem);
+
+/* ---- WINTRA module ------------------------------------------------------------
+ * EXTERNAL_MODULES.WINTRA.STATUS and SOIL_RELEASE_RATE [1/h]
+ * (read_functions.cpp:46,65).  When on, fx_advance runs the release after the
+ * ADE of every solver step (main.cpp:399-404) and the domain holds a
+ * soil_mass grid (FX_F_SOIL, zero until set).  The reference switches the
+ * module off without the ADE solver (main.cpp:207-213): status 1 on such a
+ * domain is FX_ERR_STATE.  Batches do not take WINTRA domains. */
+int fx_set_wintra(fx_domain dom, int32_t status, double soil_release_rate);
 
 /* ---- device-resident loop ----------------------------------------------------
  * Iterations of main.cpp:271-406 without host round trips: each iteration is
@@ -210,6 +224,22 @@ int fx_read_geo(int32_t nrows, int32_t ncols, double nodata, const float *dem,
  * centre, ghost extrapolation) and the initial z, h, ldry, conc grids.    */
 int fx_initiation(int32_t nrows, int32_t ncols, int32_t ade_solver, double *zb,
                   double *z, double *h, float *ldry, double *conc0);
+/* initiation with RESTART true and a loaded Results file (initiate.cpp:27-231,
+ * the load at :109-156).  table: the file's data rows, 16 doubles each, as
+ * fx_read_results returns them.  Cells the file lists take its z, h, ux, uy,
+ * qx, qy (the file holds q*dxy; the reference reads it back as the
+ * discharge, initiate.cpp:144-145), us, conc_SW[0] clamped at 0 (-1 without
+ * the ADE solver), soil_mass and twetimetracer, and ldry = 0; every other
+ * cell keeps z = zb, h = 0, zero velocities, ldry = 0 and the background soil
+ * mass (initiate.cpp:93-107).  Every output grid is overwritten.  A row
+ * naming a cell outside the padded grid is FX_ERR_ARG.                    */
+typedef struct fx_init_grids {
+    double *zb;                  /* in: read_geo's bed; out: the initial bed  */
+    double *z, *h, *qx, *qy, *us, *ux, *uy, *conc0, *soil_mass, *twet;
+    float *ldry;
+} fx_init_grids;
+int fx_initiation_restart(int32_t nrows, int32_t ncols, int32_t ade_solver, double soil_conc_bckgrd,
+                          const double *table, int64_t table_rows, fx_init_grids *grids);
 
 /* ---- files in the reference's formats (no GPU needed) ----------------------------
  * ESRI-ASCII DEM: NROWS/NCOLS from the first two header lines in either
@@ -238,8 +268,20 @@ typedef struct fx_results_grids {
     double xllcorner, yllcorner, h_min_print;
     const double *zb, *z, *h;                      /* required */
     const double *qx, *qy, *us, *conc0, *soil_mass, *fe1, *fn1, *twet;
+    const double *ux, *uy;                         /* carried from a restart file */
 } fx_results_grids;
 int fx_write_results(const char *path, const fx_results_grids *grids);
+/* A Results file read back as initial conditions (initiate.cpp:118-120):
+ * the lines after the header, 16 numbers each.  Returns the row count; with
+ * table != NULL also the rows (max_rows x 16 doubles, row-major).          */
+int fx_read_results(const char *path, int64_t max_rows, double *table, int64_t *nrows);
+/* findLastStep (common.cpp:69-106): the largest time among the folder's file
+ * names, each read as atoi of its first six characters ("3600.txt" ->
+ * 3600; the reference copies sizeof(char*)-2 bytes).  The reference skips
+ * the first two directory entries, assuming they are "." and ".."; here
+ * those two names are skipped wherever they come.  0 for an empty or
+ * missing folder. */
+int fx_find_last_step(const char *folder, int32_t *timstart);
 
 /* ---- kernel timing (extension; no counterpart in the reference) --------------
  * When enabled, every kernel the domain launches is bracketed by HIP events

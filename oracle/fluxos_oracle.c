@@ -783,7 +783,9 @@ void orc_read_geo(orc_domain *d, const float *dem, double ks_input)
 /* ------------------------------------------------------------------ */
 /* initiate.cpp:27-231, cold start                                      */
 /* ------------------------------------------------------------------ */
-void orc_initiation(orc_domain *d)
+/* initiate.cpp:42-90: ghost copies of the raw bed, then the vertex to
+ * cell-centre average over the non-(+-99999) corners */
+static void init_bed(orc_domain *d)
 {
     const int nr = d->nrows, nc = d->ncols;
     const int nr1 = nr + 1, nc1 = nc + 1;
@@ -823,30 +825,35 @@ void orc_initiation(orc_domain *d)
             zbnw = zbne;
         }
     }
-    /* initial conditions (:93-107 then the cold-start branch :166-187) */
+    free(zbs1);
+}
+
+/* initiate.cpp:93-107: depth from the (still zero) z, discharges from the
+ * (zero) velocities, background soil mass, for cells with |zb| != 99999 */
+static void init_prelude(orc_domain *d)
+{
+    const int nr = d->nrows, nc = d->ncols;
+    const double *zb = d->zb;
     for (int icol = 1; icol <= nc; icol++) {
         for (int irow = 1; irow <= nr; irow++) {
             const size_t p = IX(d, irow, icol);
             if (fabs(zb[p]) != 99999) {
                 d->h[p] = fmax(d->z[p] - zb[p], 0.0);
                 d->z[p] = zb[p] + d->h[p];
-                d->qx[p] = 0.0 * d->h[p];
-                d->qy[p] = 0.0 * d->h[p];
+                d->qx[p] = (d->ux ? d->ux[p] : 0.0) * d->h[p];
+                d->qy[p] = (d->uy ? d->uy[p] : 0.0) * d->h[p];
+                if (d->soil) d->soil[p] = d->soil_bckgrd;
             }
         }
     }
-    for (int icol = 1; icol <= nc; icol++) {
-        for (int irow = 1; irow <= nr; irow++) {
-            const size_t p = IX(d, irow, icol);
-            d->h[p] = 0.0;
-            d->z[p] = zb[p];
-            d->qx[p] = 0.0; d->qy[p] = 0.0; d->us[p] = 0.0;
-            d->twet[p] = 0.0;
-            d->ldry[p] = 1.0f;
-            d->conc[0][p] = d->ade_solver ? 0.0 : -1.0;
-        }
-    }
-    /* boundary values (:192-228) */
+}
+
+/* initiate.cpp:192-228: ghost rows then ghost columns by linear extrapolation */
+static void init_boundary(orc_domain *d)
+{
+    const int nr = d->nrows, nc = d->ncols;
+    const int nr1 = nr + 1, nc1 = nc + 1;
+    double *zb = d->zb;
     for (int icol = 0; icol <= nc1; icol++) {
         zb[IX(d, 0, icol)] = 1.5 * zb[IX(d, 1, icol)] - .5 * zb[IX(d, 2, icol)];
         zb[IX(d, nr1, icol)] = 1.5 * zb[IX(d, nr, icol)] - .5 * zb[IX(d, nr - 1, icol)];
@@ -869,7 +876,94 @@ void orc_initiation(orc_domain *d)
         d->qy[IX(d, irow, nc1)] = 0.0;
         d->qyf[IX(d, irow, 0)] = 0.0;
     }
-    free(zbs1);
+}
+
+void orc_initiation(orc_domain *d)
+{
+    const int nr = d->nrows, nc = d->ncols;
+    init_bed(d);
+    init_prelude(d);
+    /* cold-start branch (:166-187) */
+    for (int icol = 1; icol <= nc; icol++) {
+        for (int irow = 1; irow <= nr; irow++) {
+            const size_t p = IX(d, irow, icol);
+            d->h[p] = 0.0;
+            d->z[p] = d->zb[p];
+            if (d->ux) d->ux[p] = 0.0;
+            if (d->uy) d->uy[p] = 0.0;
+            d->qx[p] = 0.0; d->qy[p] = 0.0; d->us[p] = 0.0;
+            if (d->soil) d->soil[p] = 0.0;
+            d->twet[p] = 0.0;
+            d->ldry[p] = 1.0f;
+            d->conc[0][p] = d->ade_solver ? 0.0 : -1.0;
+        }
+    }
+    init_boundary(d);
+}
+
+int orc_initiation_restart(orc_domain *d, const double *table, int64_t trows)
+{
+    init_bed(d);
+    init_prelude(d);
+    /* the solute grids are allocated after the prelude (:126-129): zero */
+    const size_t mm = (size_t)d->mrows * d->mcols;
+    for (int k = 0; k < d->nchem; k++) memset(d->conc[k], 0, mm * sizeof(double));
+    /* overlay of the file's rows (:132-153); row a of the reference's
+     * filedata is table row a-1 (its row 0 is the header line) */
+    for (int64_t a = 0; a < trows; a++) {
+        const double *r = table + a * 16;
+        const unsigned irow = (unsigned)(int)r[0];
+        const unsigned icol = (unsigned)(int)r[1];
+        if (irow >= (unsigned)d->mrows || icol >= (unsigned)d->mcols) return -1;
+        const size_t p = IX(d, irow, icol);
+        d->z[p] = r[4];
+        d->h[p] = r[5];
+        if (d->ux) d->ux[p] = r[6];
+        if (d->uy) d->uy[p] = r[7];
+#ifdef ORC_PLANT_RESTART_QX
+        d->qx[p] = r[8] / (double)d->dxy;
+        d->qy[p] = r[9] / (double)d->dxy;
+#else
+        d->qx[p] = r[8];                    /* written as qx*dxy, read back as is */
+        d->qy[p] = r[9];
+#endif
+        d->us[p] = r[10];
+        d->conc[0][p] = d->ade_solver ? fmax(r[11], 0.0) : -1.0;
+        if (d->soil) d->soil[p] = r[12];
+        d->twet[p] = r[15];
+        d->ldry[p] = 0.0f;
+    }
+    init_boundary(d);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* WINTRAsolver_calc.cpp:26-57                                          */
+/* ------------------------------------------------------------------ */
+void orc_wintra(orc_domain *d, int ichem)
+{
+    /* :33-36 form f = tanh(...) from SWE_MAX, SWE_STD and the melt volume;
+     * the release below (:46-54) does not use it */
+    const int nr = d->nrows, nc = d->ncols;
+    double *c = d->conc[ichem];
+    for (int icol = 1; icol <= nc; icol++) {
+        for (int irow = 1; irow <= nr; irow++) {
+            const size_t p = IX(d, irow, icol);
+            const double hp = d->h[p];
+            const double zbp = d->zb[p];
+            if (hp > d->hdry && zbp != d->nodata) {
+#ifdef ORC_PLANT_WINTRA_AREA
+                const double deltam = d->soil[p] * d->soil_rate / 3600 * d->dtfl;
+                d->soil[p] = d->soil[p] - deltam;
+                c[p] = c[p] + deltam / hp;
+#else
+                const double deltam = d->soil[p] * d->soil_rate / 3600 * d->dtfl;
+                d->soil[p] = d->soil[p] - deltam;
+                c[p] = c[p] + deltam / (hp * (double)d->arbase);
+#endif
+            }
+        }
+    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -889,6 +983,10 @@ int orc_step(orc_domain *d, double print_next, int32_t *it, double *hpall_out)
             for (int k = 0; k < d->nchem; k++) orc_adesolver(d, *it, k, scratch);
             free(scratch);
         }
+        /* main.cpp:399-404; the module is switched off without the ADE
+         * solver (main.cpp:207-213) */
+        if (d->wintra && d->ade_solver && d->soil)
+            for (int k = 0; k < d->nchem; k++) orc_wintra(d, k);
         flags |= 1;
     }
     if (hpall_out) *hpall_out = hpall;

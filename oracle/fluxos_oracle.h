@@ -17,6 +17,8 @@
  *   orc_adesolver        <- src/fluxos/ADEsolver_calc.cpp:27-291
  *   orc_read_geo         <- src/fluxos/read_functions.cpp:112-235 (from a parsed DEM)
  *   orc_initiation       <- src/fluxos/initiate.cpp:27-231 (cold start branch)
+ *   orc_initiation_restart <- src/fluxos/initiate.cpp:27-231 (RESTART branch, :109-156)
+ *   orc_wintra           <- src/fluxos/WINTRAsolver_calc.cpp:26-57
  *   orc_step             <- src/fluxos/main.cpp:271-406 (one loop iteration)
  *
  * Layout: every grid is (nrows+2) x (ncols+2), column-major, index
@@ -43,6 +45,12 @@
  *   orc_adesolver      diffusion and advection closed forms, conservation,
  *                      zero / uniform fields, first-step skip
  *   orc_read_geo, orc_initiation   hand-worked DEMs
+ *   orc_initiation_restart  hand-written Results rows: listed cells take the
+ *                      file's values (qx/qy unscaled, conc clamped at 0,
+ *                      ldry 0), the rest the background soil mass
+ *   orc_wintra         geometric decay m (1 - r dt/3600)^n of the soil mass
+ *                      in a still wet basin, soil + solute mass conserved,
+ *                      dry and NODATA cells untouched
  * Parity unpinned (no closed form and no reference vectors): the Roe
  * dissipation with a nonzero face velocity and its |dz| limiter branches
  * (solver_wetdomain.cpp:216-229, 249-259); the stale-flux persistence of
@@ -93,6 +101,14 @@ typedef struct orc_domain {
 
     /* diagnostics */
     int64_t ub_volrat;          /* weir branch reached with volrat unset */
+
+    /* WINTRA and restart (appended; NULL / 0 = off) */
+    double *soil;               /* soil_mass grid (GlobVar.h:68) */
+    double *ux, *uy;            /* ux, uy: only carried from a Results file to the next */
+    int32_t wintra;             /* EXTERNAL_MODULES.WINTRA.STATUS */
+    int32_t pad_;
+    double soil_rate;           /* SOIL_RELEASE_RATE [1/h] */
+    double soil_bckgrd;         /* SOIL_CONC_BACKGROUND */
 } orc_domain;
 
 #ifdef __cplusplus
@@ -117,6 +133,16 @@ void orc_adesolver(orc_domain *d, int it, int ichem, double *scratch);
 void orc_read_geo(orc_domain *d, const float *dem, double ks_input);
 /* initiate.cpp:27-231, cold start (RESTART false). */
 void orc_initiation(orc_domain *d);
+/* initiate.cpp:27-231 with RESTART true and a loaded Results file.  table
+ * holds the file's data rows (the line after the header first), 16 doubles
+ * each (irow, icol, X, Y, z, h, ux, uy, qx*dxy, qy*dxy, us, conc, soil_mass,
+ * fe_1, fn_1, twet); the reference's filedata row a is table row a-1.  The
+ * grids must be zero on entry, like GlobVar's new matrices.  Returns 0, or
+ * -1 if a row names a cell outside the padded grid (the reference writes
+ * out of bounds there). */
+int orc_initiation_restart(orc_domain *d, const double *table, int64_t trows);
+/* WINTRAsolver_calc.cpp:26-57 for solute ichem. */
+void orc_wintra(orc_domain *d, int ichem);
 
 /* One iteration of main.cpp:271-406.  *it is the solver-step counter.
  * Returns bit0 = solver ran, bit1 = inflow error. */

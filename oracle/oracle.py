@@ -41,11 +41,14 @@ class OrcDomain(C.Structure):
         ("inflow_nrows", C.c_int32), ("inflow_ncols", C.c_int32), ("inflow", _F),
         ("inflow_row", C.c_int64), ("inflow_col", C.c_int64),
         ("ub_volrat", C.c_int64),
+        ("soil", _D), ("ux", _D), ("uy", _D),
+        ("wintra", C.c_int32), ("pad_", C.c_int32),
+        ("soil_rate", C.c_double), ("soil_bckgrd", C.c_double),
     ]
 
 
 DOUBLE_GRIDS = ("z", "zb", "h", "qx", "qy", "us", "ks", "fe1", "fe2", "fe3", "fn1", "fn2", "fn3",
-                "dh", "dqx", "dqy", "qxf", "qyf", "twet", "h0")
+                "dh", "dqx", "dqy", "qxf", "qyf", "twet", "h0", "soil", "ux", "uy")
 FLOAT_GRIDS = ("ldry", "weir", "ldry_prev")
 
 
@@ -86,6 +89,10 @@ def load(path: Path | None = None) -> C.CDLL:
     lib.orc_read_geo.restype = None
     lib.orc_initiation.argtypes = [P]
     lib.orc_initiation.restype = None
+    lib.orc_initiation_restart.argtypes = [P, _D, C.c_int64]
+    lib.orc_initiation_restart.restype = C.c_int
+    lib.orc_wintra.argtypes = [P, C.c_int]
+    lib.orc_wintra.restype = None
     lib.orc_step.argtypes = [P, C.c_double, C.POINTER(C.c_int32), _D]
     lib.orc_num_threads.restype = C.c_int
     _LIBS[key] = lib
@@ -189,6 +196,20 @@ class OracleDomain:
 
     def initiation(self) -> None:
         self.lib.orc_initiation(C.byref(self.s))
+
+    def initiation_restart(self, table: np.ndarray) -> None:
+        """RESTART branch of initiation; table = the Results file's data rows (n x 16)."""
+        t = np.ascontiguousarray(table, dtype=np.float64).reshape(-1, 16)
+        if self.lib.orc_initiation_restart(C.byref(self.s), t.ctypes.data_as(_D), t.shape[0]) != 0:
+            raise ValueError("restart row outside the grid")
+
+    def set_wintra(self, on: bool, soil_rate: float, soil_bckgrd: float = 0.0) -> None:
+        self.s.wintra = 1 if on else 0
+        self.s.soil_rate = float(soil_rate)
+        self.s.soil_bckgrd = float(soil_bckgrd)
+
+    def wintra(self, ichem: int = 0) -> None:
+        self.lib.orc_wintra(C.byref(self.s), int(ichem))
 
     def courant(self, print_next: float) -> float:
         return self.lib.orc_courant(C.byref(self.s), float(print_next))

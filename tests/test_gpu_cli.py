@@ -10,7 +10,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from fluxos_cpp_amd import build, gen
+from fluxos_cpp_amd import build, capi, gen
 from oracle.oracle import OracleDomain, run_reference_loop
 
 pytestmark = pytest.mark.gpu
@@ -23,9 +23,11 @@ def _expected_rows(d: OracleDomain, dxy: int, xll: float, yll: float) -> np.ndar
         for irow in range(1, d.nrows + 1):
             if g["zb"][irow, icol] != 0.0 and g["h"][irow, icol] > 0.0:     # H_MIN_TO_PRINT 0.005 -> 0
                 rows.append([irow, icol, icol * dxy + xll, irow * dxy + yll, g["z"][irow, icol],
-                             g["z"][irow, icol] - g["zb"][irow, icol], 0.0, 0.0, g["qx"][irow, icol] * dxy,
-                             g["qy"][irow, icol] * dxy, g["us"][irow, icol], d.conc[0][irow, icol], 0.0,
-                             g["fe1"][irow, icol], g["fn1"][irow, icol], g["twet"][irow, icol]])
+                             g["z"][irow, icol] - g["zb"][irow, icol], g["ux"][irow, icol],
+                             g["uy"][irow, icol], g["qx"][irow, icol] * dxy,
+                             g["qy"][irow, icol] * dxy, g["us"][irow, icol], d.conc[0][irow, icol],
+                             g["soil"][irow, icol], g["fe1"][irow, icol], g["fn1"][irow, icol],
+                             g["twet"][irow, icol]])
     return np.array(rows)
 
 
@@ -49,6 +51,57 @@ def test_cli_results_match_oracle(tmp_path):
     for t, want in expected.items():
         path = tmp_path / "Results" / f"{t}.txt"
         lines = path.read_text().splitlines()
+        got = np.array([[float(x) for x in ln.split(",")] for ln in lines[1:]])
+        assert got.shape == want.shape, (t, got.shape, want.shape)
+        assert np.array_equal(got, want), f"Results/{t}.txt differs from the oracle"
+
+
+def test_cli_restart_with_wintra(tmp_path):
+    """RESTART from the newest Results file with the WINTRA soil release on:
+    a first run writes Results/600.txt .. 1800.txt; the later files are
+    removed, so findLastStep picks 600; the second run starts its clock at 0
+    (the reference never moves ds.tim to timstart, main.cpp:255-261) from
+    600.txt's state, with background soil mass in the cells the file does not
+    list, and prints at 1200, 1800.  Compared with the oracle's restart
+    branch and loop, bit for bit."""
+    if not build.CLI.exists():
+        build.build_cli()
+    sc = gen.scenario_c1(40, seed=8, t_end=1800)
+    modset = gen.write_case(tmp_path, sc)
+    print(f"case {modset} (first run)", flush=True)
+    r = subprocess.run([str(build.CLI), str(modset)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:]
+    res = tmp_path / "Results"
+    for t in (1200, 1800):
+        (res / f"{t}.txt").unlink()
+    table = capi.read_results_file(res / "600.txt")
+    text = modset.read_text()
+    text = text.replace('"RESTART": false', '"RESTART": true')
+    text = text.replace('"WINTRA": {\n      "STATUS": false', '"WINTRA": {\n      "STATUS": true')
+    text = text.replace('"SOIL_RELEASE_RATE": 0.0', '"SOIL_RELEASE_RATE": 0.75')
+    text = text.replace('"SOIL_CONC_BACKGROUND": 0.0', '"SOIL_CONC_BACKGROUND": 3.0')
+    modset.write_text(text)
+    print("restart run", flush=True)
+    r = subprocess.run([str(build.CLI), str(modset)], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    assert "Successful loading of initial conditions file" in r.stdout and "WINTRA activated" in r.stdout
+
+    o = OracleDomain(sc.nrows, sc.ncols, sc.dxy, sc.hdry, 1, sc.d_coef, True)
+    o.read_geo(sc.dem, sc.hdry)
+    o.set_wintra(True, 0.75, 3.0)
+    o.initiation_restart(table)
+    o.set_meteo(sc.meteo)
+    expected = {}
+
+    def on_print(t, d):
+        expected[int(round(t))] = _expected_rows(d, sc.dxy, 0.0, 0.0)
+
+    run_reference_loop(o, float(sc.t_end), sc.print_step, timstart=600, on_print=on_print)
+    assert sorted(expected) == [1200, 1800]
+    assert (o.soil[1:-1, 1:-1] < 3.0).any()        # some background soil mass was released
+    for t, want in expected.items():
+        lines = (res / f"{t}.txt").read_text().splitlines()
         got = np.array([[float(x) for x in ln.split(",")] for ln in lines[1:]])
         assert got.shape == want.shape, (t, got.shape, want.shape)
         assert np.array_equal(got, want), f"Results/{t}.txt differs from the oracle"

@@ -151,7 +151,77 @@ def test_cli_rejects_unsupported_modules(tmp_path):
         build.build_cli()
     sc = gen.scenario_c1(16, seed=2, t_end=600)
     modset = gen.write_case(tmp_path, sc)
-    text = modset.read_text().replace('"RESTART": false', '"RESTART": true')
-    modset.write_text(text)
+    text = modset.read_text()
+    modset.write_text(text.replace('"OPENWQ": {\n      "STATUS": false', '"OPENWQ": {\n      "STATUS": true'))
     r = subprocess.run([str(cli), str(modset)], capture_output=True, text=True, timeout=60)
-    assert r.returncode != 0 and "RESTART" in r.stdout
+    assert r.returncode != 0 and "OPENWQ" in r.stdout
+    # a compulsory key missing is the reference's "PROBLEM loading" (read_functions.cpp:98-103)
+    modset.write_text(text.replace('"SOIL_RELEASE_RATE"', '"SOIL_RELEASE_RATE_X"'))
+    r = subprocess.run([str(cli), str(modset)], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "PROBLEM loading of master input file" in r.stdout
+
+
+# ---------------------------------------------------------------- restart ----
+def test_results_reader_round_trips_the_writer(lib, tmp_path):
+    rng = np.random.default_rng(11)
+    nr, nc, dxy = 6, 5, 2
+    shape = (nr + 2, nc + 2)
+    zb = np.asfortranarray(rng.uniform(500, 501, shape))
+    h = np.asfortranarray(rng.uniform(0.01, 2.0, shape))
+    z = np.asfortranarray(zb + h)
+    grids = {k: np.asfortranarray(rng.normal(size=shape)) for k in
+             ("qx", "qy", "conc0", "soil_mass", "fe1", "fn1", "twet", "ux", "uy")}
+    out = tmp_path / "3600.txt"
+    capi.write_results_file(out, dxy, 10.0, 20.0, 0.0, zb, z, h, **grids)
+    tab = capi.read_results_file(out)
+    assert tab.shape == (nr * nc, 16)
+    # row of cell (irow, icol) in column-major order
+    r = (3 - 1) * nr + (4 - 1)
+    assert tab[r, 0] == 4 and tab[r, 1] == 3
+    assert tab[r, 4] == z[4, 3] and tab[r, 5] == z[4, 3] - zb[4, 3]
+    assert tab[r, 6] == grids["ux"][4, 3] and tab[r, 7] == grids["uy"][4, 3]
+    assert tab[r, 8] == grids["qx"][4, 3] * dxy and tab[r, 12] == grids["soil_mass"][4, 3]
+    (tmp_path / "empty.txt").write_text("")
+    with pytest.raises(capi.FxError):
+        capi.read_results_file(tmp_path / "empty.txt")
+
+
+def test_find_last_step(lib, tmp_path):
+    assert capi.find_last_step(tmp_path / "missing") == 0
+    for name in ("0.txt", "3600.txt", "86400.txt", "7200.txt", "notes", "1234567.txt"):
+        (tmp_path / name).write_text("x")
+    # atoi of the first six characters: "1234567.txt" reads as 123456
+    assert capi.find_last_step(tmp_path) == 123456
+    (tmp_path / "1234567.txt").unlink()
+    assert capi.find_last_step(tmp_path) == 86400
+
+
+@pytest.mark.parametrize("ade", [True, False])
+def test_restart_initiation_matches_oracle(ade):
+    """fx_initiation_restart (host product code) against the oracle's
+    restatement of initiate.cpp's RESTART branch, bit for bit, on a DEM with
+    a NODATA ring and a Results table that lists a subset of cells."""
+    from oracle.oracle import OracleDomain
+    sc = gen.scenario_c1(24, seed=7)
+    rng = np.random.default_rng(3)
+    nr, nc = sc.nrows, sc.ncols
+    cells = [(i, j) for j in range(1, nc + 1) for i in range(1, nr + 1) if rng.random() < 0.3]
+    tab = np.zeros((len(cells), 16))
+    for a, (i, j) in enumerate(cells):
+        tab[a, :2] = i, j
+        tab[a, 4:] = rng.normal(size=12)
+        tab[a, 4] += 500.0
+    zb, ks, weir = capi.read_geo(sc.dem, -99999.0, sc.hdry)
+    got = capi.initiation_restart(zb, tab, ade_solver=ade, soil_conc_bckgrd=2.5)
+    o = OracleDomain(nr, nc, sc.dxy, sc.hdry, 1, [0.01], ade)
+    o.read_geo(sc.dem, sc.hdry)
+    o.set_wintra(True, 0.0, 2.5)
+    o.initiation_restart(tab)
+    pairs = {"zb": o.zb, "z": o.z, "h": o.h, "qx": o.qx, "qy": o.qy, "us": o.us, "ux": o.ux, "uy": o.uy,
+             "conc0": o.conc[0], "soil_mass": o.soil, "twet": o.twet, "ldry": o.ldry}
+    for k, want in pairs.items():
+        assert np.array_equal(got[k], want, equal_nan=True), k
+    bad = tab.copy()
+    bad[0, 1] = nc + 2
+    with pytest.raises(capi.FxError):
+        capi.initiation_restart(zb, bad)

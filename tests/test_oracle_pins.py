@@ -520,3 +520,108 @@ def test_initiation_vertex_to_centre_on_a_ramp():
         assert np.allclose(d.zb[r, 2:n - 2], 500.0 + r + 0.5, rtol=0, atol=1e-12)
     # ghosts are linear extrapolations of the first two rows (initiate.cpp:193-198)
     assert np.allclose(d.zb[0, 3:n - 2], 1.5 * d.zb[1, 3:n - 2] - 0.5 * d.zb[2, 3:n - 2])
+
+
+# ------------------------------------------------------------- WINTRA ----
+def check_wintra(lib=None):
+    """Soil release in a still basin of fixed depth: the soil mass decays
+    geometrically, m_n = m_0 (1 - r dt / 3600)^n (r per hour, dt in s), and
+    what leaves the soil arrives as solute mass c h dxy^2 in the same cell.
+    Dry cells and cells whose bed is NODATA keep both."""
+    n, dxy, H, r, m0, dt, steps = 10, 3, 0.25, 0.8, 7.0, 1.7, 25
+    d = flat_domain(n=n, dxy=dxy, h=H, lib=lib)
+    d.set_wintra(True, r, 0.0)
+    d.soil[1:-1, 1:-1] = m0
+    d.h[4, 5] = 0.5 * d.s.hdry                 # a dry cell
+    d.zb[6, 3] = d.s.nodata                    # a NODATA bed (h stays H)
+    d.dtfl = dt
+    for _ in range(steps):
+        d.wintra(0)
+    expect_soil = m0 * (1.0 - r * dt / 3600.0) ** steps
+    wet = np.ones((n + 2, n + 2), dtype=bool)
+    wet[0, :] = wet[-1, :] = wet[:, 0] = wet[:, -1] = False
+    wet[4, 5] = wet[6, 3] = False
+    assert np.allclose(d.soil[wet], expect_soil, rtol=1e-13, atol=0)
+    assert np.allclose(d.conc[0][wet], (m0 - expect_soil) / (H * dxy * dxy), rtol=1e-12, atol=0)
+    for (i, j) in ((4, 5), (6, 3)):
+        assert d.soil[i, j] == m0 and d.conc[0][i, j] == 0.0
+    # ghosts are never touched
+    assert (d.soil[0, :] == 0).all() and (d.conc[0][:, -1] == 0).all()
+
+
+def test_wintra_geometric_release():
+    check_wintra()
+
+
+def test_wintra_runs_inside_the_solver_gate_only():
+    """main.cpp:385-404: after the ADE, only on solver steps, and switched
+    off without the ADE solver (main.cpp:207-213)."""
+    for ade, h, expect_release in ((True, 0.3, True), (False, 0.3, False), (True, 0.0, False)):
+        d = flat_domain(n=10, h=h, ade=ade)
+        d.set_wintra(True, 1.5, 0.0)
+        d.soil[1:-1, 1:-1] = 4.0
+        d.conc[0][...] = 0.0
+        d.step(1e9)
+        centre = d.soil[5, 5]
+        if expect_release:
+            # the step's own dt, one release
+            assert centre == pytest.approx(4.0 * (1.0 - 1.5 * d.dtfl / 3600.0), rel=1e-14)
+            assert d.conc[0][5, 5] == pytest.approx((4.0 - centre) / (d.h[5, 5] * 4), rel=1e-12)
+        else:
+            assert centre == 4.0
+
+
+# ------------------------------------------------------------ restart ----
+def _restart_rows(dxy):
+    # irow, icol, X, Y, z, h, ux, uy, qx*dxy, qy*dxy, us, conc, soil, fe1, fn1, twet
+    return np.array([
+        [3, 4, 4 * dxy, 3 * dxy, 512.5, 2.0, 0.1, -0.2, 0.6, -0.8, 0.0, -3.0, 9.0, 0.01, 0.02, 77.0],
+        [2, 2, 2 * dxy, 2 * dxy, 511.0, 1.0, 0.0, 0.0, 0.3, 0.4, 0.0, 2.5, 1.5, 0.0, 0.0, 12.0],
+    ])
+
+
+def check_restart(lib=None):
+    """Results rows as initial conditions (initiate.cpp:109-156): listed cells
+    take the file's z, h, velocities and discharges (written as q*dxy and
+    read back without the division), the solute clamped at 0, the soil mass
+    and the wet-time tracer, and are marked wet; every other cell keeps the
+    pre-load state: water at the bed, background soil mass, solute 0."""
+    n, dxy = 6, 2
+    dem = np.full((n, n), 510.0)
+    dem[0, :] = dem[-1, :] = dem[:, 0] = dem[:, -1] = -99999.0
+    d = OracleDomain(n, n, dxy, 1e-4, 1, [0.01], True, lib_path=lib)
+    d.read_geo(dem.astype(np.float32), 1e-4)
+    d.set_wintra(True, 0.5, 5.0)
+    d.initiation_restart(_restart_rows(dxy))
+    assert (d.z[3, 4], d.h[3, 4], d.ux[3, 4], d.uy[3, 4]) == (512.5, 2.0, 0.1, -0.2)
+    assert (d.qx[3, 4], d.qy[3, 4]) == (0.6, -0.8)
+    assert d.conc[0][3, 4] == 0.0 and d.conc[0][2, 2] == 2.5
+    assert (d.soil[3, 4], d.soil[2, 2], d.twet[3, 4]) == (9.0, 1.5, 77.0)
+    assert d.ldry[3, 4] == 0.0 and d.ldry[2, 2] == 0.0
+    # an unlisted interior cell
+    assert d.z[4, 3] == d.zb[4, 3] and d.h[4, 3] == 0.0
+    assert d.soil[4, 3] == 5.0 and d.conc[0][4, 3] == 0.0 and d.ldry[4, 3] == 0.0
+    # ghost rows follow the loaded interior (initiate.cpp:193-199)
+    assert d.z[0, 4] == 1.5 * d.z[1, 4] - 0.5 * d.z[2, 4]
+
+
+def test_restart_overlay():
+    check_restart()
+
+
+def test_restart_without_ade_marks_solute_absent():
+    n, dxy = 6, 2
+    dem = np.full((n, n), 510.0)
+    d = OracleDomain(n, n, dxy, 1e-4, 1, [0.01], False)
+    d.read_geo(dem.astype(np.float32), 1e-4)
+    d.initiation_restart(_restart_rows(dxy))
+    assert d.conc[0][3, 4] == -1.0 and d.conc[0][4, 3] == 0.0
+
+
+def test_restart_row_outside_grid_is_refused():
+    d = OracleDomain(6, 6, 2, 1e-4, 1, [0.01], True)
+    d.read_geo(np.full((6, 6), 510.0, dtype=np.float32), 1e-4)
+    rows = _restart_rows(2)
+    rows[1, 0] = 9
+    with pytest.raises(ValueError):
+        d.initiation_restart(rows)

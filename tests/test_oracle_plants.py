@@ -28,6 +28,8 @@ PLANTS = [
     ("QUICK", pins.check_ade_advection),          # upwind-biased face weights 3/4, 3/8 -> 7/8, 1/4
     ("DRY_BED", pins.check_dry_face_pressure),    # face bed = the dry cell's own bed, not the mean
     ("CONV", pins.check_uniform_flow),            # x-momentum carried with v instead of u
+    ("WINTRA_AREA", pins.check_wintra),           # released mass divided by h only, not h dxy^2
+    ("RESTART_QX", pins.check_restart),           # q*dxy read back divided by dxy
 ]
 
 
