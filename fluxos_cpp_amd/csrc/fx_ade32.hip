@@ -44,8 +44,9 @@ struct apref {
     double dold, dnew;
     float fe, fn, mult;
     float m[NC];
-    uint32_t f;
+    uint32_t f;               // flag byte, plus AF_INFLOW
 };
+#define AF_INFLOW 0x100u      // the cell takes the inflow point source
 
 // per-cell geometry of one column, shared by the solutes
 struct ageo {
@@ -59,8 +60,9 @@ struct actx {
     const double *d_old, *d_new;
     const float *m_in[ANC];
     float *m_out[ANC];
-    double meteoi;
-    float cm[ANC], kdiff[ANC];
+    double meteoi, inflowi;
+    uint32_t inflow_p;        // linear index of the inflow cell, ~0u = none
+    float cm[ANC], cin[ANC], kdiff[ANC];
     float hdry, dx, area_dt, dt, area;
     int row, mcols;
     uint32_t ld;
@@ -80,6 +82,7 @@ __device__ inline void a_load(const actx &X, int c, apref<NC> &a)
     a.fe = A.ff[p];
     a.fn = A.ff[(uint32_t)A.mm + p];
     a.mult = X.P->melt_mult ? X.P->melt_mult[p] : 1.0f;
+    if (p == X.inflow_p) a.f |= AF_INFLOW;
 #pragma unroll
     for (int k = 0; k < NC; k++) a.m[k] = X.m_in[k][p];
 }
@@ -98,7 +101,14 @@ __device__ inline void a_derive(const actx &X, const apref<NC> &a, ageo &g, floa
     const double hn = fmax(0.0, a.dnew);
     const bool w = inner && !(hn < P.hdry) && !weir;
     const bool wp = inner && !(fmax(0.0, a.dold) < P.hdry) && !weir;
-    const float h0 = inner ? (float)fmax(0.0, a.dold + melt) : 0.f;
+    double dm = a.dold + melt;
+    float inf = 0.f;                                   // inflow depth carrying solute
+    if (a.f & AF_INFLOW) {                             // add_inflow (common.cpp:324-376)
+        if (dm <= 0.0) dm = 0.0;
+        if (dm > 0.0) inf = (float)X.inflowi;          // solute arrives only into water
+        dm += X.inflowi;
+    }
+    const float h0 = inner ? (float)fmax(0.0, dm) : 0.f;
     const float mlt = (inner && a.dold > 0.0) ? (float)melt : 0.f;   // solute arrives with melt only into water
     g.h = inner ? (float)hn : 0.f;
     g.w = w;
@@ -106,7 +116,7 @@ __device__ inline void a_derive(const actx &X, const apref<NC> &a, ageo &g, floa
     g.fn = inner ? a.fn : 0.f;
 #pragma unroll
     for (int k = 0; k < NC; k++) {
-        const float mv = inner ? a.m[k] + mlt * X.cm[k] : 0.f;
+        const float mv = inner ? a.m[k] + mlt * X.cm[k] + inf * X.cin[k] : 0.f;
         float c = 0.f, r = 0.f;
         if (h0 > 0.f || w) {
             r = h0 > 0.f ? mv / h0 : 0.f;
@@ -251,6 +261,8 @@ __device__ inline void ade_tile(const fx_params &P, const a32p &A, fx_ctl *ctl, 
     X.d_new = A.d[cur ^ 1];
     X.transport = ctl->gate && ctl->it > 1;
     X.meteoi = P.meteo ? ctl->meteoi : 0.0;
+    X.inflowi = ctl->inflowi;
+    X.inflow_p = P.inflow ? (uint32_t)P.inflow_p : ~0u;
     const float dt = (float)ctl->dtfl;
     X.hdry = (float)P.hdry;
     X.dx = (float)P.dxy;
@@ -262,6 +274,7 @@ __device__ inline void ade_tile(const fx_params &P, const a32p &A, fx_ctl *ctl, 
         X.m_in[k] = A.mass[cur][A.k0 + k];
         X.m_out[k] = A.mass[cur ^ 1][A.k0 + k];
         X.cm[k] = (float)ctl->cmelt[A.k0 + k];
+        X.cin[k] = (float)ctl->cin[A.k0 + k];
         X.kdiff[k] = 2.f * (float)P.D[A.k0 + k];     // 1/sigc/dx*dy with sigc = 0.5, dx = dy
     }
     const int lane = threadIdx.x;

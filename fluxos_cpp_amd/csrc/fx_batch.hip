@@ -218,6 +218,7 @@ int fx_batch_admit(fx_batch b, fx_domain d, double t_end, double print_next, int
     if (d->desc.precision != FX_FP32_MIXED) return fx_fail(FX_ERR_UNSUPPORTED, "batches hold FP32-mixed domains");
     if (!d->has_static) return fx_fail(FX_ERR_STATE, "fx_set_static has not been called");
     if (d->wintra) return fx_fail(FX_ERR_UNSUPPORTED, "WINTRA domains run alone (fx_advance)");
+    if ((rc = fx_inflow_cell_ok(d))) return rc;
     const int dade = (d->desc.ade_solver && d->desc.ade_mode == FX_ADE_FAST) ? 1 : 0;
     if (dade != b->ade || (dade && d->desc.nchem != b->nchem))
         return fx_fail(FX_ERR_ARG, "the domain's solutes differ from the batch's");

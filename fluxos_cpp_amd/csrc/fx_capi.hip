@@ -107,7 +107,21 @@ fx_params fx_make_params(fx_domain d)
     P.cvdef = (double)D.cvdef;
     P.nuem = (double)D.nuem;
     for (int k = 0; k < FX_MAX_CHEM; k++) P.D[k] = D.d_coef[k];
+    if (D.precision == FX_FP32_MIXED) {
+        P.d32[0] = d->s32.d[0];
+        P.d32[1] = d->s32.d[1];
+    }
     return P;
+}
+
+// FP32-mixed domains check the inflow cell's bed on the host (the FP64 kernels
+// flag it as they go, common.cpp:324,377-380)
+int fx_inflow_cell_ok(fx_domain d)
+{
+    if (d->desc.precision != FX_FP32_MIXED || !d->inflow || d->zb_host.empty()) return FX_OK;
+    const size_t p = (size_t)d->inflow_irow + (size_t)d->inflow_icol * d->mrows;
+    if (d->zb_host[p] == d->desc.nodata) return fx_fail(FX_ERR_ARG, "inflow ix,iy in NODATA_VALUE cell");
+    return FX_OK;
 }
 
 namespace {
@@ -576,8 +590,8 @@ int fx_set_inflow(fx_domain d, int32_t nrows, int32_t ncols, const float *table,
         if (ncols < 2) return fx_fail(FX_ERR_ARG, "inflow table needs at least 2 columns");
         if (irow < 0 || irow >= d->mrows || icol < 0 || icol >= d->mcols)
             return fx_fail(FX_ERR_ARG, "inflow cell outside the grid");
-        if (d->desc.precision != FX_FP64_PARITY)
-            return fx_fail(FX_ERR_UNSUPPORTED, "the inflow point source runs on FP64 parity domains");
+        if (irow < 1 || irow > d->desc.nrows || icol < 1 || icol > d->desc.ncols)
+            return fx_fail(FX_ERR_UNSUPPORTED, "inflow cell on the ghost ring: no interior cell receives it");
         FX_HIP(hipMalloc((void **)&d->inflow, (size_t)nrows * ncols * sizeof(float)));
         FX_HIP(hipMemcpy(d->inflow, table, (size_t)nrows * ncols * sizeof(float), hipMemcpyHostToDevice));
         d->inflow_nrows = nrows;
@@ -681,6 +695,7 @@ int fx_advance(fx_domain d, double t_end, double print_next, int32_t max_steps, 
         info->status = 2;
         return FX_OK;
     }
+    if ((rc = fx_inflow_cell_ok(d))) return rc;
     if (d->desc.precision == FX_FP32_MIXED && (rc = fx32_launch_cfl_initial(d))) return rc;
     if (d->desc.precision == FX_FP32_MIXED && (rc = pull_ctl(d))) return rc;
     c->print_next = print_next;
@@ -706,7 +721,7 @@ int fx_advance(fx_domain d, double t_end, double print_next, int32_t max_steps, 
             if ((rc = launch_step(d))) return rc;
         if ((rc = pull_ctl(d))) return rc;
         done = c->steps;
-        if (c->halt || (c->status & 4) || c->ade_error) break;
+        if (c->halt || (c->status & 12) || c->ade_error) break;
         dt_est = c->dtfl;
         tim0 = c->tim;
     }
@@ -718,6 +733,12 @@ int fx_advance(fx_domain d, double t_end, double print_next, int32_t max_steps, 
     info->solver_steps = c->solver_steps;
     info->status = (c->tim >= print_next ? 1 : 0) | (c->tim > t_end ? 2 : 0) | (c->status & 4);
     if (c->ade_error) return fx_fail(FX_ERR_HIP, "ADE wavefront wait expired");
+    if (c->status & 8) {
+        c->status &= ~8;
+        if ((rc = push_ctl(d))) return rc;
+        FX_HIP(hipStreamSynchronize(d->stream));
+        return fx_fail(FX_ERR_ARG, "inflow ix,iy in NODATA_VALUE cell");
+    }
     if (c->status & 4) return fx_fail(FX_ERR_NONFINITE, "non-finite value in the state");
     return FX_OK;
 }

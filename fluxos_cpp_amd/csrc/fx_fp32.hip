@@ -272,8 +272,9 @@ __device__ inline void cfl_cell(const fx_params &P, double h, float qx, float qy
 struct pref {                 // one cell's HBM values in flight
     double d;
     float qx, qy, zbh, zbl, mult;
-    uint32_t f;
+    uint32_t f;               // flag byte, plus PF_INFLOW
 };
+#define PF_INFLOW 0x100u      // the cell takes the inflow point source
 
 struct ring {
     float dm[3][NS], qx[3][NS], qy[3][NS], u[3][NS], v[3][NS], sq[3][NS], zbh[3][NS], zbl[3][NS];
@@ -292,7 +293,8 @@ struct sweep_ctx {
     const float *qx_in, *qy_in;
     float *qx_out, *qy_out;
     consts32 K;
-    double meteoi, s_wet_prev;
+    double meteoi, inflowi, s_wet_prev;
+    uint32_t inflow_p;        // linear index of the inflow cell, ~0u = none
     float dxy_over_dt, kf, dtf, hdryf;
     int gate, ade_on, k, row, mcols;
     bool row_ok, flux_lane, own;
@@ -312,6 +314,7 @@ __device__ inline void load_col(const sweep_ctx &X, int c, pref &a)
     a.zbh = S.zb[p];
     a.zbl = S.zb_lo[p];
     a.mult = X.P->melt_mult ? X.P->melt_mult[p] : 1.0f;
+    if (p == X.inflow_p) a.f |= PF_INFLOW;
 }
 
 // L: this lane's cell of the column into ring slot SL (melt of this step,
@@ -323,6 +326,10 @@ __device__ inline void store_col(const sweep_ctx &X, ring &R, const pref &a)
     const bool inner = a.f & FXF_INNER;
     double d = inner ? a.d : 0.0;
     if (inner && !(a.f & FXF_NODATA)) d += X.meteoi * (double)a.mult;
+    if (a.f & PF_INFLOW) {                              // add_inflow after add_meteo (common.cpp:324-330)
+        if (d <= 0.0) d = 0.0;
+        d += X.inflowi;
+    }
     const bool wet = d > X.P->hdry;                     // loop 1: h = max(0,d) > hdry
     float qx = inner ? a.qx : 0.f, qy = inner ? a.qy : 0.f;
     if (X.gate && !wet) { qx = 0.f; qy = 0.f; }
@@ -500,6 +507,8 @@ __device__ inline void sweep_tile(const fx_params &P, const s32p &S, fx_ctl *ctl
     X.gate = ctl->gate;
     X.ade_on = ade_on;
     X.meteoi = P.meteo ? ctl->meteoi : 0.0;
+    X.inflowi = ctl->inflowi;
+    X.inflow_p = P.inflow ? (uint32_t)P.inflow_p : ~0u;
     X.s_wet_prev = ctl->s_wet_prev;
     X.k = threadIdx.x;
     const int r0 = 1 + bx * TH;
@@ -667,6 +676,21 @@ __device__ inline void clock_advance(const fx_params &P, fx_ctl *ctl, double a, 
 __device__ inline void clock_forcing(const fx_params &P, fx_ctl *ctl, int mrow, int irow)
 {
     fx_forcing_eval(P, ctl, mrow, irow);
+    if (P.inflow) {                  // the inflow cell's bed is never NODATA here (checked by the host)
+        // water in: the source, plus a deficit of the cell filled up to the
+        // bed first (common.cpp:324-325), judged after this step's melt
+        double vin = ctl->inflowi;
+        if (P.d32[0]) {
+            const size_t p = (size_t)P.inflow_p;
+            double dm = P.d32[ctl->cur][p];
+            if (P.meteo) dm += ctl->meteoi * (P.melt_mult ? (double)P.melt_mult[p] : 1.0);
+            if (dm <= 0.0) vin -= dm;
+        }
+        const double y = vin * P.arbase - ctl->led_melt_c;
+        const double t = ctl->led_melt + y;
+        ctl->led_melt_c = (t - ctl->led_melt) - y;
+        ctl->led_melt = t;
+    }
     if (P.meteo) {
         const double y = ctl->meteoi * ctl->melt_area * P.arbase - ctl->led_melt_c;
         const double t = ctl->led_melt + y;

@@ -150,6 +150,7 @@ int fx_launch_forcing(fx_domain d);
 int fx_launch_step_end(fx_domain d);
 int fx64_launch_volume(fx_domain d, double *vol_out);
 int fx64_launch_wintra(fx_domain d, unsigned mode, int k_first, int k_count);
+int fx_inflow_cell_ok(fx_domain d);
 int fx32_launch_wintra(fx_domain d);
 
 int fx32_init_state(fx_domain d);

@@ -28,6 +28,7 @@ struct fx_params {
     const float *meteo;
     const float *melt_mult;
     const float *inflow;
+    const double *d32[2];     // FP32-mixed depth planes (the clock's inflow ledger), else NULL
 };
 
 fx_params fx_make_params(fx_domain d);

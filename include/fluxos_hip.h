@@ -125,7 +125,9 @@ int fx_get_clock(fx_domain dom, double *tim, double *dtfl, int32_t *it);
 int fx_set_meteo(fx_domain dom, int32_t nrows, int32_t ncols, const float *table,
                  const float *melt_mult);
 /* Inflow table as read by read_inflow (read_functions.cpp:285-326): rows
- * from 0, and the discharge cell (read_functions.cpp:228-231).            */
+ * from 0, and the discharge cell (read_functions.cpp:228-231), an interior
+ * cell.  fx_advance returns FX_ERR_ARG if the cell's bed is NODATA_VALUE
+ * (common.cpp:377-380).                                                   */
 int fx_set_inflow(fx_domain dom, int32_t nrows, int32_t ncols, const float *table,
                   int64_t irow, int64_t icol);
 
@@ -163,9 +165,11 @@ int fx_set_wintra(fx_domain dom, int32_t status, double soil_release_rate);
  * (main.cpp:444) or after max_steps iterations.                           */
 int fx_advance(fx_domain dom, double t_end, double print_next, int32_t max_steps,
                fx_step_info *info);
-/* Water volume of the non-weir cells and the running ledger of melt input and
- * weir-face exchange, all accumulated in FP64 (extension; no counterpart in
- * the reference).  out[0]=volume, out[1]=melt in, out[2]=net weir inflow. */
+/* Water volume of the non-weir cells and the running ledger of forcing input
+ * and weir-face exchange, all accumulated in FP64 (extension; no counterpart
+ * in the reference).  out[0]=volume, out[1]=melt and inflow in (with the
+ * inflow cell's deficit filled up to the bed, common.cpp:324-325),
+ * out[2]=net weir inflow.  The ledger is kept by FP32-mixed domains. */
 int fx_mass_balance(fx_domain dom, double out[3]);
 /* Tile shape of the FP32-mixed sweeps (extension; no counterpart in the
  * reference): columns walked per block by the step sweep (ch) and by the

@@ -114,3 +114,82 @@ def test_fp32_ade_fast_bounds():
           f"mean {dc.mean():.3e} mg/L over {wet.sum()} wet cells", flush=True)
     assert np.percentile(dc, 99) < 0.01
     assert dc.max() < 0.05
+
+
+def _inflow_table():
+    # t [s], discharge [m3/s], solute [mg/L]: the first row later than t applies
+    return np.array([[0, 0.0, 0.0], [300, 0.8, 5.0], [900, 0.4, 2.0], [3600, 0.1, 1.0]], dtype=np.float32)
+
+
+def test_fp32_inflow_short_span_and_mass_balance():
+    """add_inflow (common.cpp:277-390) on the FP32-mixed path: the point source
+    fills its cell (a deficit first clamped to the bed) before the step, like
+    the FP64 path; depth within the short-span bar, and the FP64 ledger
+    (melt plus inflow in, weir exchange) closes the water balance."""
+    sc = gen.scenario_c1(64, seed=3, t_end=3600)
+    init, d64, d32 = _pair(sc, ade=True)
+    tab = _inflow_table()
+    irow, icol = 30, 33
+    for d in (d64, d32):
+        d.set_inflow(tab, irow, icol)
+    v0 = d32.mass_balance()[0]
+    i64 = _advance_to(d64, 420.0)
+    i32 = _advance_to(d32, 420.0)
+    h64, h32 = d64.get("h"), d32.get("h")
+    err = float(np.abs(h64 - h32)[1:-1, 1:-1].max())
+    c64, c32 = d64.get_conc(0)[irow, icol], d32.get_conc(0)[irow, icol]
+    print(f"t={i64.tim:.4f}/{i32.tim:.4f} h(inflow)={h64[irow, icol]:.6f}/{h32[irow, icol]:.6f} "
+          f"max|dh|={err:.3e} c(inflow)={c64:.6f}/{c32:.6f}", flush=True)
+    assert h64[irow, icol] > 0.01                       # the source has filled its cell
+    assert err <= H_TOL
+    assert abs(c32 - c64) <= 1e-3 * max(1.0, abs(c64))
+    _advance_to(d32, 1800.0)
+    v, fin, weir = d32.mass_balance()
+    rel = abs(v - v0 - fin - weir) / max(v, 1e-300)
+    print(f"V0={v0:.9e} V={v:.9e} in={fin:.9e} weir={weir:.9e} rel err={rel:.3e}", flush=True)
+    assert rel <= MB_TOL
+
+
+def test_fp32_inflow_pointwise_after_solver_steps():
+    """A moderate source (1.5 m into its cell over the first, print-clipped
+    step, then a dam-break front): depth and solute within the short-span
+    bar over the first ~10 solver steps, before the front's wet/dry
+    thresholds amplify rounding."""
+    sc = gen.scenario_c1(64, seed=3, t_end=3600)
+    init, d64, d32 = _pair(sc, ade=True)
+    tab = np.array([[0, 0.0, 0.0], [900, 0.01, 4.0], [3600, 0.005, 1.0]], dtype=np.float32)
+    irow, icol = 30, 33
+    for d in (d64, d32):
+        d.set_inflow(tab, irow, icol)
+    i64 = _advance_to(d64, 605.0)
+    i32 = _advance_to(d32, 605.0)
+    h64, h32 = d64.get("h"), d32.get("h")
+    err = float(np.abs(h64 - h32)[1:-1, 1:-1].max())
+    c64, c32 = d64.get_conc(0), d32.get_conc(0)
+    wet = (d64.get("ldry") == 0) & (d32.get("ldry") == 0)
+    dc = np.abs(c64 - c32)[wet]
+    print(f"it={i64.it}/{i32.it} h(inflow)={h64[irow, icol]:.6f}/{h32[irow, icol]:.6f} max|dh|={err:.3e} "
+          f"c(inflow)={c64[irow, icol]:.5f}/{c32[irow, icol]:.5f} |dc| p99 {np.percentile(dc, 99):.3e} "
+          f"max {dc.max():.3e}", flush=True)
+    assert i64.it == i32.it and i64.it >= 5
+    assert err <= H_TOL
+    # fast ADE against the exact sweep, scaled to the source's 4 mg/L (as for the melt's 1 mg/L)
+    assert np.percentile(dc, 99) < 0.04 and dc.max() < 0.2
+
+
+def test_inflow_on_nodata_bed_is_an_error():
+    """The reference stops with "inflow ix,iy in NODATA_VALUE cell"
+    (common.cpp:377-380) when the source cell's bed equals NODATA_VALUE;
+    both paths report it from fx_advance."""
+    sc = gen.scenario_c1(32, seed=1)
+    init = capi.prepare(sc)
+    zb = init.zb.copy()
+    r, c = 10, 12
+    zb[r, c] = -99999.0
+    for prec in (capi.FP64_PARITY, capi.FP32_MIXED):
+        d = capi.HipDomain.from_scenario(sc, precision=prec, init=init)
+        d.set_static(zb, init.ks, init.weir)
+        d.set_inflow(_inflow_table(), r, c)
+        with pytest.raises(capi.FxError, match="NODATA"):
+            d.advance(3600.0, 600.0)
+        d.close()
