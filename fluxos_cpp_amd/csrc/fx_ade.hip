@@ -12,9 +12,12 @@
 //    one lane down and moves up with one cross-lane shift, no LDS, no barrier;
 //  * a 64-lane wave covers 64 columns; consecutive waves are chained through
 //    global memory: the producer publishes its last lane's progress every few
-//    rows and the consumer waits (bounded) before it needs a row.  Waves take
-//    column blocks in ticket order (atomic counter), so a wave only ever
-//    waits on a wave that started earlier.
+//    rows, and the consumer fetches the W boundary (the producer's north-face
+//    flux and final concentration) ADE_HK rows at a time, one block ahead of
+//    use, after a bounded wait for the rows it needs; so no step of the sweep
+//    waits on a memory round trip.  Waves take column blocks in ticket order
+//    (atomic counter), so a wave only ever waits on a wave that started
+//    earlier.
 //
 // Pass 1 (the neighbour bounds and the rescale to the new depth) is a plain
 // parallel kernel writing the rescaled copy B; the sweep reads B for cells
@@ -24,6 +27,7 @@
 
 #define ADE_W 64
 #define ADE_PUBLISH 8
+#define ADE_HK 8              // W-boundary rows fetched per batch (<= ADE_W, a multiple of ADE_PUBLISH)
 #define ADE_SPIN_LIMIT 20000000LL
 
 namespace {
@@ -155,34 +159,59 @@ __global__ __launch_bounds__(ADE_W) void k_ade_sweep(fx_params P, fx_state64 s, 
         arow_load(X, k - lane + 1, ring[k]);
     }
 
+    // W boundary of lane 0 (column b*ADE_W: the previous wave's last column,
+    // or ghost column 0 with qfcds = 0): lane j of cur / nxt holds row
+    // base + j of the block in use / of the next block
+    const size_t wcol = (size_t)b * ADE_W * ld;
+    double cur_q = 0.0, cur_c = 0.0, nxt_q = 0.0, nxt_c = 0.0;
+    auto fetch = [&](int r0) -> bool {
+        if (r0 > nr) return true;
+        if (b > 0) {
+            const uint32_t need = (uint32_t)min(r0 + ADE_HK - 1, nr);
+            if (lane == 0 && seen < need) {
+                long long spin = 0;
+                for (;;) {
+                    const uint32_t pr = __hip_atomic_load(&progress[b - 1], __ATOMIC_ACQUIRE,
+                                                          __HIP_MEMORY_SCOPE_AGENT);
+                    if (pr >= need) { seen = pr; break; }
+                    if (++spin > ADE_SPIN_LIMIT) { bail = true; break; }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            if (__shfl(bail ? 1 : 0, 0, ADE_W)) return false;
+        }
+        const int r = r0 + lane;
+        if (lane < ADE_HK && r <= nr) {
+            nxt_q = b > 0 ? __hip_atomic_load(&qedge[(size_t)(b - 1) * ld + r], __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT)
+                          : 0.0;
+            nxt_c = __hip_atomic_load(&A[(size_t)r + wcol], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        return true;
+    };
+    if (!fetch(1)) bail = true;
+    cur_q = nxt_q;
+    cur_c = nxt_c;
+    if (!bail && !fetch(1 + ADE_HK)) bail = true;
+    if (bail) {
+        if (lane == 0) atomicOr(&ctl->ade_error, 1u);
+        return;
+    }
+
     const int T = nr + ADE_W - 1;
     auto step = [&](int t, arow &R) {
+        // lane 0 works on row t + 1: a new block of the W boundary every ADE_HK steps
+        if (t > 0 && t % ADE_HK == 0) {
+            cur_q = nxt_q;
+            cur_c = nxt_c;
+            if (!fetch(t + 1 + ADE_HK)) return false;
+        }
+        const double qb = __shfl(cur_q, t % ADE_HK, ADE_W), cb = __shfl(cur_c, t % ADE_HK, ADE_W);
         // receive the S neighbour's (qfn, final c) for this lane's row
         double q_in = shift_up(q_out), c_in = shift_up(c_out);
         const int ix = t - lane + 1;
         const bool active = col_ok && ix >= 1 && ix <= nr;
-        if (lane == 0 && active) {
-            if (b == 0) {
-                q_in = 0.0;                                  // qfcds starts at zero
-                c_in = A[(size_t)ix];                        // ghost column 0
-            } else {
-                if ((uint32_t)ix > seen) {
-                    long long spin = 0;
-                    for (;;) {
-                        uint32_t pr = __hip_atomic_load(&progress[b - 1], __ATOMIC_ACQUIRE,
-                                                        __HIP_MEMORY_SCOPE_AGENT);
-                        if (pr >= (uint32_t)ix) { seen = pr; break; }
-                        if (++spin > ADE_SPIN_LIMIT) { bail = true; break; }
-                        __builtin_amdgcn_s_sleep(1);
-                    }
-                }
-                q_in = __hip_atomic_load(&qedge[(size_t)(b - 1) * ld + ix], __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
-                c_in = __hip_atomic_load(&A[(size_t)ix + (size_t)(iy - 1) * ld], __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
-            }
-        }
-        if (__shfl(bail ? 1 : 0, 0, ADE_W)) return false;
+        if (lane == 0) { q_in = qb; c_in = cb; }
         if (active) {
             const size_t p = (size_t)ix + (size_t)iy * ld;
             const double Bee = R.Bee;

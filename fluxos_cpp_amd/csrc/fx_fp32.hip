@@ -738,15 +738,23 @@ __global__ __launch_bounds__(1024) void k32_clock(fx_params P, double *partials,
     }
     __syncthreads();
     if (!last) return;
+    // second level, by the whole block: the other blocks' part2 entries are
+    // read past this CU's cache (volatile), one entry per thread
+    __threadfence();
     double A = 9.e10, B = 0.0, Cw = 0.0;
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const volatile double *q = part2;
-        for (int k = 0; k < (int)gridDim.x; k++) {
-            A = fmin(A, q[3 * k]);
-            B = fmax(B, q[3 * k + 1]);
-            Cw += q[3 * k + 2];
-        }
+    for (int k = threadIdx.x; k < (int)gridDim.x; k += blockDim.x) {
+        const volatile double *q = part2 + 3 * (size_t)k;
+        A = fmin(A, q[0]);
+        B = fmax(B, q[1]);
+        Cw += q[2];
+    }
+    __shared__ double sm2[3][16];
+    A = wmin(A); B = wmax(B); Cw = wsum(Cw);
+    const int tid = threadIdx.x;
+    if ((tid & 63) == 0) { sm2[0][tid >> 6] = A; sm2[1][tid >> 6] = B; sm2[2][tid >> 6] = Cw; }
+    __syncthreads();
+    if (tid == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) { A = fmin(A, sm2[0][w]); B = fmax(B, sm2[1][w]); Cw += sm2[2][w]; }
         *ticket = 0;
     }
     clock_finish(P, ctl, A, B, Cw);
