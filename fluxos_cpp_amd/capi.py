@@ -420,8 +420,10 @@ class HipDomain:
         _check(self.lib.fx_mass_balance(self.h, out))
         return out[0], out[1], out[2]
 
-    def profile(self, on: bool) -> None:
-        _check(self.lib.fx_profile(self.h, 1 if on else 0))
+    def profile(self, on: bool, every: int = 1) -> None:
+        """Kernel timing with HIP events on the domain's stream: every launch,
+        or one launch in `every` of each kernel class."""
+        _check(self.lib.fx_profile(self.h, max(1, int(every)) if on else 0))
 
     def profile_read(self, kernel: int):
         ms, n = C.c_double(), C.c_int64()

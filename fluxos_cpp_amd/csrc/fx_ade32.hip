@@ -323,9 +323,10 @@ __device__ inline void ade_tile(const fx_params &P, const a32p &A, fx_ctl *ctl, 
 }
 
 template <int NC>
-__global__ __launch_bounds__(64) void k32_ade_sweep(fx_params P, a32p A, fx_ctl *ctl)
+__global__ __launch_bounds__(64) void k32_ade_sweep(fx_params P, a32p A, fx_ctl *ctl, int gx, int n, int remap)
 {
-    ade_tile<NC>(P, A, ctl, blockIdx.x, blockIdx.y);
+    const int t = fx_xcd_tile((int)blockIdx.x, n, remap);
+    ade_tile<NC>(P, A, ctl, t % gx, t / gx);
 }
 
 // batched: block b runs block FX_BMAP_BLK(amap[b]) of member FX_BMAP_SLOT(amap[b]);
@@ -345,7 +346,9 @@ __global__ __launch_bounds__(64) void k32b_ade_sweep(const fx_bslot *slots, cons
 template <int NC>
 void launch_nc(fx_domain d, const fx_params &P, a32p A, dim3 g)
 {
-    hipLaunchKernelGGL(k32_ade_sweep<NC>, g, dim3(64), 0, d->stream, P, A, d->ctl);
+    const int n = (int)(g.x * g.y);
+    hipLaunchKernelGGL(k32_ade_sweep<NC>, dim3(n), dim3(64), 0, d->stream, P, A, d->ctl, (int)g.x, n,
+                       fx32_xcd_remap() ? 1 : 0);
 }
 
 inline dim3 ade_grid(const fx_params &P)

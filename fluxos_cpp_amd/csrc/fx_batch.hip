@@ -92,6 +92,18 @@ int rebuild(fx_batch b)
         if (B.ade_on)
             for (int k = 0; k < b->nab[s]; k++) a.push_back(FX_BMAP(s, k));
     }
+    // workgroup w runs entry fx_xcd_tile(w) of the member-ordered list: each
+    // XCD takes a contiguous run of tiles
+    if (fx32_xcd_remap()) {
+        auto permute = [](std::vector<uint32_t> &v) {
+            std::vector<uint32_t> o(v.size());
+            const int n = (int)v.size();
+            for (int w = 0; w < n; w++) o[w] = v[fx_xcd_tile(w, n, 1)];
+            v.swap(o);
+        };
+        permute(t);
+        permute(a);
+    }
     int rc;
     if ((rc = grow(&b->tmap, &b->tmap_cap, std::max<int64_t>(1, (int64_t)t.size())))) return rc;
     if ((rc = grow(&b->amap, &b->amap_cap, std::max<int64_t>(1, (int64_t)a.size())))) return rc;

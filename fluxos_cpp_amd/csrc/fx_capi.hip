@@ -32,6 +32,8 @@ void fx_prof_begin(fx_domain d, int id)
 {
     if (!d->prof.on) return;
     fx_prof &p = d->prof;
+    p.skipping = id >= 0 && id < FX_K_COUNT && (p.seq[id]++ % p.stride) != 0;
+    if (p.skipping) return;
     if (p.pool.size() < 2) {
         for (int i = 0; i < 64; i++) {
             hipEvent_t e;
@@ -50,7 +52,7 @@ void fx_prof_begin(fx_domain d, int id)
 void fx_prof_end(fx_domain d, int id)
 {
     (void)id;
-    if (!d->prof.on || d->prof.pending.empty()) return;
+    if (!d->prof.on || d->prof.pending.empty() || d->prof.skipping) return;
     (void)hipEventRecord(d->prof.pending.back().b, d->stream);
     if (d->prof.pending.size() > 8192) (void)fx_prof_drain(d);
 }
@@ -785,8 +787,10 @@ int fx_profile(fx_domain d, int32_t on)
     int rc = enter(d);
     if (rc) return rc;
     if ((rc = fx_prof_drain(d))) return rc;
-    for (int i = 0; i < FX_K_COUNT; i++) { d->prof.total_ms[i] = 0.0; d->prof.count[i] = 0; }
+    for (int i = 0; i < FX_K_COUNT; i++) { d->prof.total_ms[i] = 0.0; d->prof.count[i] = 0; d->prof.seq[i] = 0; }
     d->prof.on = on != 0;
+    d->prof.stride = on > 1 ? on : 1;
+    d->prof.skipping = false;
     return FX_OK;
 }
 

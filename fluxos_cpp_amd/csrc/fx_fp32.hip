@@ -27,6 +27,8 @@
 // the face fluxes see the bed as the FP64 path does.
 #include "fx_kern32.h"
 
+#include <cstdlib>
+
 #pragma clang fp contract(fast)
 
 
@@ -562,10 +564,12 @@ __device__ inline void sweep_tile(const fx_params &P, const s32p &S, fx_ctl *ctl
     }
 }
 
-__global__ __launch_bounds__(64) void k32_sweep(fx_params P, s32p S, fx_ctl *ctl, int ade_on)
+__global__ __launch_bounds__(64) void k32_sweep(fx_params P, s32p S, fx_ctl *ctl, int ade_on, int gx, int n,
+                                                int remap)
 {
     __shared__ ring R;
-    sweep_tile(P, S, ctl, ade_on, blockIdx.x, blockIdx.y, gridDim.x, R);
+    const int t = fx_xcd_tile((int)blockIdx.x, n, remap);
+    sweep_tile(P, S, ctl, ade_on, t % gx, t / gx, gx, R);
 }
 
 // batched: block b runs block FX_BMAP_BLK(tmap[b]) of member FX_BMAP_SLOT(tmap[b])
@@ -845,6 +849,8 @@ inline dim3 grid32(const fx_params &P)
 {
     return dim3((P.nrows + TH - 1) / TH, (P.ncols + P.ch - 1) / P.ch);
 }
+
+
 
 } // namespace
 
@@ -1163,6 +1169,15 @@ int fx32_launch_cfl_initial(fx_domain d)
     return fx_hip_check(hipGetLastError(), "k32_cfl");
 }
 
+bool fx32_xcd_remap()
+{
+    static const bool on = [] {
+        const char *e = std::getenv("FLUXOS_XCD_REMAP");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 int fx32_launch_wintra(fx_domain d)
 {
     fx_params P = fx_make_params(d);
@@ -1182,7 +1197,10 @@ int fx32_launch_step(fx_domain d)
     FX_HIP(hipGetLastError());
     const int ade_on = (d->desc.ade_solver && d->desc.ade_mode == FX_ADE_FAST) ? 1 : 0;
     fx_prof_begin(d, FX_K_STEP32);
-    hipLaunchKernelGGL(k32_sweep, grid32(P), dim3(64), 0, d->stream, P, S, d->ctl, ade_on);
+    const dim3 g = grid32(P);
+    const int n = (int)(g.x * g.y);
+    hipLaunchKernelGGL(k32_sweep, dim3(n), dim3(64), 0, d->stream, P, S, d->ctl, ade_on, (int)g.x, n,
+                       fx32_xcd_remap() ? 1 : 0);
     fx_prof_end(d, FX_K_STEP32);
     FX_HIP(hipGetLastError());
     if (ade_on) {

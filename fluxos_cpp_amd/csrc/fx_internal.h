@@ -90,6 +90,9 @@ struct fx_state32 {
 // per-kernel HIP-event timing on the domain's stream (fx_profile)
 struct fx_prof {
     bool on = false;
+    int stride = 1;            // time one launch in `stride` of each kernel class
+    bool skipping = false;     // the open begin/end pair is not timed
+    int64_t seq[FX_K_COUNT] = {0};
     struct rec { int id; hipEvent_t a, b; };
     std::vector<rec> pending;
     std::vector<hipEvent_t> pool;

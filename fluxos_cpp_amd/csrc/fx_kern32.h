@@ -70,6 +70,20 @@ int fx32b_launch_sweep(hipStream_t st, const fx_bslot *slots, const uint32_t *tm
 int fx32b_launch_ade(hipStream_t st, const fx_bslot *slots, const uint32_t *amap, int nblocks, int nchem);
 int fx32b_launch_end(hipStream_t st, const fx_bslot *slots, int nslot, fx_bstat *stat);
 
+// Tile of workgroup w of an n-tile launch.  Workgroups go to the eight XCDs
+// round-robin (w % 8), and each XCD has its own L2; with remap, XCD x takes
+// one contiguous run of tiles (strips fastest), so the column-block
+// neighbour whose halo columns a tile re-reads was processed shortly before
+// on the same XCD.  A bijection of [0, n) for every n.
+__host__ __device__ inline int fx_xcd_tile(int w, int n, int remap)
+{
+    if (!remap) return w;
+    const int x = w & 7, q = n >> 3, r = n & 7;
+    return x * q + (x < r ? x : r) + (w >> 3);
+}
+// XCD-aware tile order on (FLUXOS_XCD_REMAP=0 turns it off)
+bool fx32_xcd_remap();
+
 #define FX_BMAP(slot, blk) (((uint32_t)(slot) << 20) | (uint32_t)(blk))
 #define FX_BMAP_SLOT(v) ((v) >> 20)
 #define FX_BMAP_BLK(v) ((v) & 0xFFFFFu)

@@ -288,9 +288,11 @@ int fx_read_results(const char *path, int64_t max_rows, double *table, int64_t *
 int fx_find_last_step(const char *folder, int32_t *timstart);
 
 /* ---- kernel timing (extension; no counterpart in the reference) --------------
- * When enabled, every kernel the domain launches is bracketed by HIP events
- * recorded on the domain's own stream.  fx_profile_read returns the summed
- * device time [ms] and launch count of one kernel class since the last reset. */
+ * When enabled (on >= 1), kernels the domain launches are bracketed by HIP
+ * events recorded on the domain's own stream: every launch with on = 1, one
+ * launch in `on` of each kernel class otherwise (less event overhead).
+ * fx_profile_read returns the summed device time [ms] and count of the timed
+ * launches of one kernel class since the last reset. */
 #define FX_K_CFL        0   /* FP64 CFL reduction                          */
 #define FX_K_MELT       1   /* FP64 forcing + hydro loop 1                 */
 #define FX_K_FLUX       2   /* FP64 solver_dry / solver_wet                */
