@@ -323,9 +323,9 @@ __device__ inline void ade_tile(const fx_params &P, const a32p &A, fx_ctl *ctl, 
 }
 
 template <int NC>
-__global__ __launch_bounds__(64) void k32_ade_sweep(fx_params P, a32p A, fx_ctl *ctl, int gx, int n, int remap)
+__global__ __launch_bounds__(64) void k32_ade_sweep(fx_params P, a32p A, fx_ctl *ctl, int gx, int n, int chunk)
 {
-    const int t = fx_xcd_tile((int)blockIdx.x, n, remap);
+    const int t = fx_xcd_tile((int)blockIdx.x, n, chunk);
     ade_tile<NC>(P, A, ctl, t % gx, t / gx);
 }
 
@@ -348,7 +348,7 @@ void launch_nc(fx_domain d, const fx_params &P, a32p A, dim3 g)
 {
     const int n = (int)(g.x * g.y);
     hipLaunchKernelGGL(k32_ade_sweep<NC>, dim3(n), dim3(64), 0, d->stream, P, A, d->ctl, (int)g.x, n,
-                       fx32_xcd_remap() ? 1 : 0);
+                       fx32_xcd_chunk((int)g.x, n));
 }
 
 inline dim3 ade_grid(const fx_params &P)

@@ -92,13 +92,13 @@ int rebuild(fx_batch b)
         if (B.ade_on)
             for (int k = 0; k < b->nab[s]; k++) a.push_back(FX_BMAP(s, k));
     }
-    // workgroup w runs entry fx_xcd_tile(w) of the member-ordered list: each
-    // XCD takes a contiguous run of tiles
-    if (fx32_xcd_remap()) {
-        auto permute = [](std::vector<uint32_t> &v) {
+    // workgroup w runs entry fx_xcd_tile(w) of the member-ordered list (runs of
+    // tiles dealt to the XCDs)
+    if (const int chunk = fx32_xcd_chunk_batch()) {
+        auto permute = [chunk](std::vector<uint32_t> &v) {
             std::vector<uint32_t> o(v.size());
             const int n = (int)v.size();
-            for (int w = 0; w < n; w++) o[w] = v[fx_xcd_tile(w, n, 1)];
+            for (int w = 0; w < n; w++) o[w] = v[fx_xcd_tile(w, n, chunk)];
             v.swap(o);
         };
         permute(t);

@@ -71,18 +71,31 @@ int fx32b_launch_ade(hipStream_t st, const fx_bslot *slots, const uint32_t *amap
 int fx32b_launch_end(hipStream_t st, const fx_bslot *slots, int nslot, fx_bstat *stat);
 
 // Tile of workgroup w of an n-tile launch.  Workgroups go to the eight XCDs
-// round-robin (w % 8), and each XCD has its own L2; with remap, XCD x takes
-// one contiguous run of tiles (strips fastest), so the column-block
-// neighbour whose halo columns a tile re-reads was processed shortly before
-// on the same XCD.  A bijection of [0, n) for every n.
-__host__ __device__ inline int fx_xcd_tile(int w, int n, int remap)
+// round-robin (w % 8), and each XCD has its own L2.  With chunk > 0 the tiles
+// are dealt to the XCDs in runs of `chunk` consecutive tiles (strips fastest),
+// round-robin: a tile's column-block neighbour, whose halo columns it
+// re-reads, is usually in the same run, so on the same XCD and processed
+// shortly before.  The tail that does not fill a whole round goes in one
+// contiguous run per XCD.  A bijection of [0, n) for every n and chunk.
+__host__ __device__ inline int fx_xcd_tile(int w, int n, int chunk)
 {
-    if (!remap) return w;
-    const int x = w & 7, q = n >> 3, r = n & 7;
-    return x * q + (x < r ? x : r) + (w >> 3);
+    if (chunk <= 0) return w;
+    const int round = 8 * chunk;
+    const int nfull = (n / round) * round;
+    if (w >= nfull) {
+        const int m = n - nfull, v = w - nfull;
+        const int x = v & 7, q = m >> 3, r = m & 7;
+        return nfull + x * q + (x < r ? x : r) + (v >> 3);
+    }
+    const int x = w & 7, l = w >> 3;                 // l-th workgroup of XCD x
+    return ((l / chunk) * 8 + x) * chunk + l % chunk;
 }
-// XCD-aware tile order on (FLUXOS_XCD_REMAP=0 turns it off)
-bool fx32_xcd_remap();
+// run length of the tile deal of a single domain's sweep with gx strips per
+// column block and n tiles (FLUXOS_XCD_COLS = column blocks per run, default
+// 4; 0 = no remap; negative = one run per XCD), and of a batch's tile maps
+// (FLUXOS_XCD_BATCH = tiles per run; unset or 0 = no remap)
+int fx32_xcd_chunk(int gx, int n);
+int fx32_xcd_chunk_batch();
 
 #define FX_BMAP(slot, blk) (((uint32_t)(slot) << 20) | (uint32_t)(blk))
 #define FX_BMAP_SLOT(v) ((v) >> 20)
