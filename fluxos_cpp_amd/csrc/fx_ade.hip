@@ -34,7 +34,7 @@ namespace {
 
 // pass 1 over the whole padded grid: ghosts copied, interior rescaled
 // (ADEsolver_calc.cpp:64-72).  Also resets the wavefront's counters.
-__global__ __launch_bounds__(256) void k_ade_pass1(fx_params P, fx_state64 s, int ichem,
+__global__ __launch_bounds__(256) void k_ade_pass1(fx_params P, fx_state64 s, int ichem, int slot,
                                                    fx_ctl *ctl, unsigned mode,
                                                    uint32_t *progress, int nblocks)
 {
@@ -43,10 +43,10 @@ __global__ __launch_bounds__(256) void k_ade_pass1(fx_params P, fx_state64 s, in
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i == 0) ctl->ade_ticket = 0;
     if (blockIdx.x == 0)
-        for (int b = threadIdx.x; b < nblocks; b += 256) progress[b] = 0;
+        for (int b = threadIdx.x; b < nblocks; b += 256) progress[(size_t)slot * nblocks + b] = 0;
     if (i >= mm) return;
     const double *A = s.conc[ichem];
-    double *B = s.conc_tmp;
+    double *B = s.conc_tmp + (size_t)slot * mm;
     const int ix = (int)(i % P.ld), iy = (int)(i / P.ld);
     double c = A[i];
     if (ix >= 1 && ix <= P.nrows && iy >= 1 && iy <= P.ncols) {
@@ -105,20 +105,30 @@ __device__ inline void arow_load(const actx64 &X, int r, arow &a)
             (X.ldry[p + ld] == 1 ? 4u : 0u) | (X.ldry[pnn] == 1 ? 8u : 0u);
 }
 
-#define ADE_DEPTH 4
+#ifndef ADE_DEPTH
+#define ADE_DEPTH 4           // rows of loads in flight ahead of use
+#endif
 
-__global__ __launch_bounds__(ADE_W) void k_ade_sweep(fx_params P, fx_state64 s, int ichem,
+// nsol solutes (k0 .. k0+nsol-1) sweep concurrently, each its own chain of
+// waves: ticket t is block t / nsol of solute slot t % nsol, so a wave only
+// waits on a wave of its own solute with a smaller ticket
+__global__ __launch_bounds__(ADE_W) void k_ade_sweep(fx_params P, fx_state64 s, int k0, int nsol,
                                                      fx_ctl *ctl, unsigned mode,
-                                                     uint32_t *progress, double *qedge)
+                                                     uint32_t *progress_all, double *qedge_all, int nblocks)
 {
     if ((mode & FXM_GATED) && (ctl->halt || !ctl->gate || ctl->it <= 1)) return;
     const int lane = threadIdx.x;
     uint32_t ticket = 0;
     if (lane == 0) ticket = atomicAdd(&ctl->ade_ticket, 1u);
     ticket = __shfl(ticket, 0, ADE_W);
-    const int b = (int)ticket;
+    const int slot = (int)ticket % nsol;
+    const int b = (int)ticket / nsol;
+    const int ichem = k0 + slot;
     const int nr = P.nrows, nc = P.ncols;
     const size_t ld = P.ld;
+    const int64_t mm = (int64_t)ld * (nc + 2);
+    uint32_t *progress = progress_all + (size_t)slot * nblocks;
+    double *qedge = qedge_all + (size_t)slot * nblocks * ld;
     const int iy = b * ADE_W + 1 + lane;
     const bool col_ok = iy <= nc;
     double *A = s.conc[ichem];
@@ -127,7 +137,7 @@ __global__ __launch_bounds__(ADE_W) void k_ade_sweep(fx_params P, fx_state64 s, 
     const double dtfl = ctl->dtfl, D = P.D[ichem];
     const double nt = 1, sigc = 0.5;
     actx64 X;
-    X.A = A; X.B = s.conc_tmp; X.h = s.h; X.fe1 = s.fe1; X.fn1 = s.fn1; X.ldry = ldry;
+    X.A = A; X.B = s.conc_tmp + (size_t)slot * mm; X.h = s.h; X.fe1 = s.fe1; X.fn1 = s.fn1; X.ldry = ldry;
     X.ld = ld; X.nr = nr; X.nc = nc; X.iy = iy; X.col_ok = col_ok;
     X.inn = (iy + 2 < nc + 1) ? iy + 2 : nc + 1;
 
@@ -154,6 +164,7 @@ __global__ __launch_bounds__(ADE_W) void k_ade_sweep(fx_params P, fx_state64 s, 
         f0 = X.fe1[c0];
     }
     arow ring[ADE_DEPTH];
+#pragma unroll
     for (int k = 0; k < ADE_DEPTH; k++) {      // rows of steps 0 .. ADE_DEPTH-1
         ring[k] = arow{};
         arow_load(X, k - lane + 1, ring[k]);
@@ -345,14 +356,13 @@ __global__ __launch_bounds__(ADE_W) void k_ade_sweep(fx_params P, fx_state64 s, 
     bool ok = true;
     int t = 0;
     for (; ok && t + ADE_DEPTH <= T; t += ADE_DEPTH) {
-        ok = step(t, ring[0]);
-        if (ok) ok = step(t + 1, ring[1]);
-        if (ok) ok = step(t + 2, ring[2]);
-        if (ok) ok = step(t + 3, ring[3]);
+#pragma unroll
+        for (int k = 0; k < ADE_DEPTH; k++)
+            if (ok) ok = step(t + k, ring[k]);
     }
-    if (ok && t < T) ok = step(t, ring[0]);
-    if (ok && t + 1 < T) ok = step(t + 1, ring[1]);
-    if (ok && t + 2 < T) ok = step(t + 2, ring[2]);
+#pragma unroll
+    for (int k = 0; k < ADE_DEPTH - 1; k++)
+        if (ok && t + k < T) ok = step(t + k, ring[k]);
     if (!ok) {
         if (lane == 0) atomicOr(&ctl->ade_error, 1u);
         return;
@@ -366,22 +376,25 @@ __global__ __launch_bounds__(ADE_W) void k_ade_sweep(fx_params P, fx_state64 s, 
 
 } // namespace
 
-int fx64_launch_ade_exact(fx_domain d, unsigned mode, int ichem, int it_override)
+int fx64_launch_ade_exact(fx_domain d, unsigned mode, int k0, int nsol)
 {
-    (void)it_override;
     fx_params P = fx_make_params(d);
     const int64_t mm = (int64_t)d->mrows * d->mcols;
     const int nblocks = (P.ncols + ADE_W - 1) / ADE_W;
-    if (nblocks > d->ade_blocks) return fx_fail(FX_ERR_STATE, "ADE block buffers too small");
-    fx_prof_begin(d, FX_K_ADE_PASS1);
-    hipLaunchKernelGGL(k_ade_pass1, dim3((unsigned)((mm + 255) / 256)), dim3(256), 0, d->stream,
-                       P, d->s64, ichem, d->ctl, mode, d->ade_progress, nblocks);
-    fx_prof_end(d, FX_K_ADE_PASS1);
-    FX_HIP(hipGetLastError());
-    double *qedge = (double *)(d->ade_progress + ((d->ade_blocks + 63) / 64) * 64);
+    if (nblocks > d->ade_blocks || nsol < 1 || k0 < 0 || k0 + nsol > d->desc.nchem)
+        return fx_fail(FX_ERR_STATE, "ADE block buffers too small");
+    for (int k = 0; k < nsol; k++) {
+        fx_prof_begin(d, FX_K_ADE_PASS1);
+        hipLaunchKernelGGL(k_ade_pass1, dim3((unsigned)((mm + 255) / 256)), dim3(256), 0, d->stream,
+                           P, d->s64, k0 + k, k, d->ctl, mode, d->ade_progress, nblocks);
+        fx_prof_end(d, FX_K_ADE_PASS1);
+        FX_HIP(hipGetLastError());
+    }
+    const int64_t nb = (int64_t)d->ade_blocks * d->desc.nchem;
+    double *qedge = (double *)(d->ade_progress + ((nb + 63) / 64) * 64);
     fx_prof_begin(d, FX_K_ADE_SWEEP);
-    hipLaunchKernelGGL(k_ade_sweep, dim3(nblocks), dim3(ADE_W), 0, d->stream,
-                       P, d->s64, ichem, d->ctl, mode, d->ade_progress, qedge);
+    hipLaunchKernelGGL(k_ade_sweep, dim3(nblocks * nsol), dim3(ADE_W), 0, d->stream,
+                       P, d->s64, k0, nsol, d->ctl, mode, d->ade_progress, qedge, nblocks);
     fx_prof_end(d, FX_K_ADE_SWEEP);
     return fx_hip_check(hipGetLastError(), "k_ade_sweep");
 }

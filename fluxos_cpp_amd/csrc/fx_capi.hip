@@ -140,7 +140,7 @@ int64_t fp64_bytes(const fx_domain_desc &D)
     const int64_t mm = (int64_t)(D.nrows + 2) * (D.ncols + 2);
     int64_t b = mm * 8 * 14 + mm * 3;
     b += mm * 8 * std::max(1, D.nchem);
-    if (D.ade_solver) b += mm * 8;
+    if (D.ade_solver) b += mm * 8 * std::max(1, D.nchem);    // pass-1 copies, one per solute
     return b;
 }
 
@@ -270,9 +270,9 @@ int launch_step64(fx_domain d)
     if ((rc = fx64_launch_melt(d, G | FXM_MELT | FXM_INFLOW | FXM_LOOP1))) return rc;
     if ((rc = fx64_launch_flux(d, G))) return rc;
     if ((rc = fx64_launch_update(d, G))) return rc;
-    if (d->desc.ade_solver && d->desc.ade_mode != FX_ADE_OFF)
-        for (int k = 0; k < d->desc.nchem; k++)
-            if ((rc = fx64_launch_ade_exact(d, G, k, 0))) return rc;
+    if (d->desc.ade_solver && d->desc.ade_mode != FX_ADE_OFF &&
+        (rc = fx64_launch_ade_exact(d, G, 0, d->desc.nchem)))          // the solutes are independent
+        return rc;
     if (d->wintra && (rc = fx64_launch_wintra(d, G, 0, d->desc.nchem))) return rc;
     return fx_launch_step_end(d);
 }
@@ -340,9 +340,12 @@ int fx_domain_create(const fx_domain_desc *desc, fx_domain *out)
         for (int k = 0; k < d->desc.nchem; k++)
             if ((rc = dalloc(d, &s.conc[k], mm))) return bail(rc);
         if (d->desc.ade_solver) {
-            if ((rc = dalloc(d, &s.conc_tmp, mm))) return bail(rc);
+            // pass-1 copies and wavefront buffers per solute: the solutes of a
+            // step sweep concurrently
+            if ((rc = dalloc(d, &s.conc_tmp, mm * d->desc.nchem))) return bail(rc);
             d->ade_blocks = (d->desc.ncols + 63) / 64;
-            const int64_t words = ((d->ade_blocks + 63) / 64) * 64 + 2 * (int64_t)d->ade_blocks * d->mrows;
+            const int64_t nb = (int64_t)d->ade_blocks * d->desc.nchem;
+            const int64_t words = ((nb + 63) / 64) * 64 + 2 * nb * d->mrows;
             if ((rc = dalloc(d, &d->ade_progress, words))) return bail(rc);
         }
         d->bytes = fp64_bytes(d->desc);
@@ -674,7 +677,7 @@ int fx_adesolver_calc(fx_domain d, int32_t it, int32_t ichem)
     if (!d->desc.ade_solver) return fx_fail(FX_ERR_STATE, "the ADE solver is off for this domain");
     if (ichem < 0 || ichem >= d->desc.nchem) return fx_fail(FX_ERR_ARG, "ichem out of range");
     if (it <= 1) return FX_OK;                      // ADEsolver_calc.cpp:45,74-77
-    if ((rc = fx64_launch_ade_exact(d, 0, ichem, it))) return rc;
+    if ((rc = fx64_launch_ade_exact(d, 0, ichem, 1))) return rc;
     if ((rc = pull_ctl(d))) return rc;
     if (d->ctl_host->ade_error) return fx_fail(FX_ERR_HIP, "ADE wavefront wait expired");
     return FX_OK;

@@ -148,7 +148,7 @@ int fx64_launch_cfl(fx_domain d, unsigned mode);
 int fx64_launch_melt(fx_domain d, unsigned mode);
 int fx64_launch_flux(fx_domain d, unsigned mode);
 int fx64_launch_update(fx_domain d, unsigned mode);
-int fx64_launch_ade_exact(fx_domain d, unsigned mode, int ichem, int it_override);
+int fx64_launch_ade_exact(fx_domain d, unsigned mode, int k0, int nsol);   // solutes k0 .. k0+nsol-1
 int fx_launch_forcing(fx_domain d);
 int fx_launch_step_end(fx_domain d);
 int fx64_launch_volume(fx_domain d, double *vol_out);
