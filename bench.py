@@ -237,7 +237,7 @@ def main() -> int:
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the path is a "
-              "single-device kernel; multi-GPU sharding is described in DESIGN.md", file=sys.stderr)
+              "single-device kernel (row-strip sharding across GPUs is not built)", file=sys.stderr)
         return 2
 
     from fluxos_cpp_amd import capi, gen
