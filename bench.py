@@ -300,7 +300,11 @@ def main() -> int:
     # melt input and weir-face exchange
     v, melt, weir = d32.mass_balance()
     mass = {"rel_err": abs(v - v0 - melt - weir) / max(abs(v), 1e-300), "volume_m3": v,
-            "melt_m3": melt, "weir_m3": weir}
+            "melt_m3": melt, "weir_m3": weir,
+            # solute kept in the non-weir cells against the FP64 ledger of solute the
+            # melt brought into water (the reference's scheme zeroes it in cells that
+            # dry and in weir cells, so the two differ by those sinks)
+            "solute": [dict(zip(("mass_g", "brought_in_g"), d32.solute_balance(k))) for k in range(sc.nchem)]}
 
     # roofline of the dominant kernel (algorithmic bytes: hydro 28 B per
     # cell-update, ADE 8 B per solute per cell-update)

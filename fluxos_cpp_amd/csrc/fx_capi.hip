@@ -785,6 +785,32 @@ int fx_mass_balance(fx_domain d, double out[3])
     return FX_OK;
 }
 
+int fx_solute_balance(fx_domain d, int32_t ichem, double out[2])
+{
+    int rc = enter(d);
+    if (rc) return rc;
+    if (!out) return fx_fail(FX_ERR_ARG, "out is NULL");
+    if (!d->desc.ade_solver) return fx_fail(FX_ERR_STATE, "the ADE solver is off for this domain");
+    if (ichem < 0 || ichem >= d->desc.nchem) return fx_fail(FX_ERR_ARG, "ichem out of range");
+    double mass = 0.0;
+    if (d->desc.precision == FX_FP64_PARITY) {
+        if ((rc = fx64_launch_solute(d, ichem, &mass))) return rc;
+        out[0] = mass;
+        out[1] = 0.0;
+        return FX_OK;
+    }
+    if ((rc = fx32_launch_solute(d, ichem, &mass))) return rc;
+    double area = 0.0;
+    if ((rc = fx32_pending(d, nullptr, &area))) return rc;
+    if ((rc = pull_ctl(d))) return rc;
+    const fx_ctl *c = d->ctl_host;
+    double in = c->sled[ichem];
+    if (d->meteo) in += c->meteoi * area * (double)d->desc.dxy * d->desc.dxy * c->cmelt[ichem];
+    out[0] = mass;
+    out[1] = in;
+    return FX_OK;
+}
+
 int fx_profile(fx_domain d, int32_t on)
 {
     int rc = enter(d);

@@ -322,10 +322,13 @@ __device__ inline void load_col(const sweep_ctx &X, int c, pref &a)
 // L: this lane's cell of the column into ring slot SL (melt of this step,
 // hydro loop 1, cell velocities)
 template <int SL>
-__device__ inline void store_col(const sweep_ctx &X, ring &R, const pref &a)
+__device__ inline void store_col(const sweep_ctx &X, ring &R, const pref &a, bool own_col, double &ms)
 {
     const int i = X.k;
     const bool inner = a.f & FXF_INNER;
+    // melt area that brings solute (cells holding water before the melt,
+    // common.cpp:214-233), summed over the cells this lane owns
+    if (own_col && X.own && inner && !(a.f & FXF_NODATA) && a.d > 0.0) ms += (double)a.mult;
     double d = inner ? a.d : 0.0;
     if (inner && !(a.f & FXF_NODATA)) d += X.meteoi * (double)a.mult;
     if (a.f & PF_INFLOW) {                              // add_inflow after add_meteo (common.cpp:324-330)
@@ -478,11 +481,11 @@ __device__ inline void update_col(const sweep_ctx &X, ring &R, int c, double &dt
 // one column of the sweep; SC = ring slot of column c
 template <int SC>
 __device__ inline void column_step(const sweep_ctx &X, ring &R, int c, int c_last, pref &a,
-                                   double &dtm, double &hmx, double &wv)
+                                   double &dtm, double &hmx, double &wv, double &ms)
 {
     constexpr int SN = (SC + 1) % 3, SS = (SC + 2) % 3;
     __syncthreads();
-    store_col<SN>(X, R, a);                           // column c+1 (prefetched last step)
+    store_col<SN>(X, R, a, c + 1 <= c_last, ms);      // column c+1 (prefetched last step)
     if (c + 2 <= c_last + 1) load_col(X, c + 2, a);
     __syncthreads();
     flux_col<SC, SS, SN>(X, R, c);
@@ -536,31 +539,33 @@ __device__ inline void sweep_tile(const fx_params &P, const s32p &S, fx_ctl *ctl
     X.dtf = ctl->k_dt;
 
     pref a;
-    double dtm = 9.e10, hmx = 0.0, wv = 0.0;
+    double dtm = 9.e10, hmx = 0.0, wv = 0.0, ms = 0.0;
     // prologue: columns c0-2 (slot 1), c0-1 (slot 2), c0 (slot 0), fluxes of c0-1
-    load_col(X, c0 - 2, a); store_col<1>(X, R, a);
-    load_col(X, c0 - 1, a); store_col<2>(X, R, a);
-    load_col(X, c0, a); store_col<0>(X, R, a);
+    load_col(X, c0 - 2, a); store_col<1>(X, R, a, false, ms);
+    load_col(X, c0 - 1, a); store_col<2>(X, R, a, false, ms);
+    load_col(X, c0, a); store_col<0>(X, R, a, true, ms);
     load_col(X, c0 + 1, a);
     __syncthreads();
     flux_col<2, 1, 0>(X, R, c0 - 1);
     // column c uses ring slot (c - c0) % 3, unrolled by three
     int c = c0;
     for (; c + 2 <= c_last; c += 3) {
-        column_step<0>(X, R, c, c_last, a, dtm, hmx, wv);
-        column_step<1>(X, R, c + 1, c_last, a, dtm, hmx, wv);
-        column_step<2>(X, R, c + 2, c_last, a, dtm, hmx, wv);
+        column_step<0>(X, R, c, c_last, a, dtm, hmx, wv, ms);
+        column_step<1>(X, R, c + 1, c_last, a, dtm, hmx, wv, ms);
+        column_step<2>(X, R, c + 2, c_last, a, dtm, hmx, wv, ms);
     }
-    if (c <= c_last) { column_step<0>(X, R, c, c_last, a, dtm, hmx, wv); c++; }
-    if (c <= c_last) { column_step<1>(X, R, c, c_last, a, dtm, hmx, wv); c++; }
+    if (c <= c_last) { column_step<0>(X, R, c, c_last, a, dtm, hmx, wv, ms); c++; }
+    if (c <= c_last) { column_step<1>(X, R, c, c_last, a, dtm, hmx, wv, ms); c++; }
     dtm = wmin(dtm);
     hmx = wmax(hmx);
     wv = wsum(wv);
+    ms = wsum(ms);
     if (X.k == 0) {
         const size_t bid = bx + (size_t)by * gx;
-        S.partials[3 * bid] = dtm;
-        S.partials[3 * bid + 1] = hmx;
-        S.partials[3 * bid + 2] = wv;
+        S.partials[FX32_NPV * bid] = dtm;
+        S.partials[FX32_NPV * bid + 1] = hmx;
+        S.partials[FX32_NPV * bid + 2] = wv;
+        S.partials[FX32_NPV * bid + 3] = ms;
     }
 }
 
@@ -603,8 +608,8 @@ __global__ __launch_bounds__(64) void k32_cfl(fx_params P, s32p S, fx_ctl *ctl)
     hmx = wmax(hmx);
     if (k == 0) {
         const size_t bid = blockIdx.x + (size_t)blockIdx.y * gridDim.x;
-        S.partials[3 * bid] = dtm;
-        S.partials[3 * bid + 1] = hmx;
+        S.partials[FX32_NPV * bid] = dtm;
+        S.partials[FX32_NPV * bid + 1] = hmx;
     }
 }
 
@@ -614,34 +619,39 @@ __global__ __launch_bounds__(64) void k32_cfl(fx_params P, s32p S, fx_ctl *ctl)
 //
 // clock_fold: one block's min dt / max hpall / weir-volume sum over partials
 // [lo, hi), in a fixed order (the weir entries are consumed: set to 0).
-__device__ inline void clock_fold(double *partials, int lo, int hi, double &a, double &b, double &c)
+__device__ inline void clock_fold(double *partials, int lo, int hi, double &a, double &b, double &c, double &m)
 {
     const int tid = threadIdx.x;
-    a = 9.e10; b = 0.0; c = 0.0;
+    a = 9.e10; b = 0.0; c = 0.0; m = 0.0;
     for (int i = lo + tid; i < hi; i += blockDim.x) {
-        a = fmin(a, partials[3 * (size_t)i]);
-        b = fmax(b, partials[3 * (size_t)i + 1]);
-        c += partials[3 * (size_t)i + 2];
-        partials[3 * (size_t)i + 2] = 0.0;
+        double *q = partials + FX32_NPV * (size_t)i;
+        a = fmin(a, q[0]);
+        b = fmax(b, q[1]);
+        c += q[2];
+        m += q[3];
+        q[2] = 0.0;
+        q[3] = 0.0;
     }
-    __shared__ double sm[3][16];
-    a = wmin(a); b = wmax(b); c = wsum(c);
-    if ((tid & 63) == 0) { sm[0][tid >> 6] = a; sm[1][tid >> 6] = b; sm[2][tid >> 6] = c; }
+    __shared__ double sm[4][16];
+    a = wmin(a); b = wmax(b); c = wsum(c); m = wsum(m);
+    if ((tid & 63) == 0) { sm[0][tid >> 6] = a; sm[1][tid >> 6] = b; sm[2][tid >> 6] = c; sm[3][tid >> 6] = m; }
     __syncthreads();
     if (tid == 0)
-        for (int w = 1; w < (int)(blockDim.x >> 6); w++) { a = fmin(a, sm[0][w]); b = fmax(b, sm[1][w]); c += sm[2][w]; }
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+            a = fmin(a, sm[0][w]); b = fmax(b, sm[1][w]); c += sm[2][w]; m += sm[3][w];
+        }
 }
 
-__device__ inline void clock_advance(const fx_params &P, fx_ctl *ctl, double a, double b, double c);
+__device__ inline void clock_advance(const fx_params &P, fx_ctl *ctl, double a, double b, double c, double m);
 __device__ inline void clock_forcing(const fx_params &P, fx_ctl *ctl, int mrow, int irow);
 
 // clock_finish: the step's clock from the folded (dt, hpall, weir).  Every
 // thread of the block calls it (the forcing-table lookup is shared); thread 0
 // holds the folded values and writes the control block.
-__device__ inline void clock_finish(const fx_params &P, fx_ctl *ctl, double a, double b, double c)
+__device__ inline void clock_finish(const fx_params &P, fx_ctl *ctl, double a, double b, double c, double m)
 {
     __shared__ double tim_new;
-    if (threadIdx.x == 0) clock_advance(P, ctl, a, b, c);
+    if (threadIdx.x == 0) clock_advance(P, ctl, a, b, c, m);
     if (threadIdx.x == 0) tim_new = ctl->tim;
     __syncthreads();
     const int mrow = P.meteo ? fx_forcing_row_block(P.meteo, P.meteo_nrows, P.meteo_ncols, tim_new) : -1;
@@ -649,15 +659,23 @@ __device__ inline void clock_finish(const fx_params &P, fx_ctl *ctl, double a, d
     if (threadIdx.x == 0) clock_forcing(P, ctl, mrow, irow);
 }
 
-__device__ inline void clock_advance(const fx_params &P, fx_ctl *ctl, double a, double b, double c)
+__device__ inline void kahan_add(double &sum, double &comp, double v)
+{
+    const double y = v - comp;
+    const double t = sum + y;
+    comp = (t - sum) - y;
+    sum = t;
+}
+
+__device__ inline void clock_advance(const fx_params &P, fx_ctl *ctl, double a, double b, double c, double m)
 {
     // Kahan-compensated ledger of the weir-face exchange of the previous step
-    {
-        const double y = c * P.arbase - ctl->led_weir_c;
-        const double t = ctl->led_weir + y;
-        ctl->led_weir_c = (t - ctl->led_weir) - y;
-        ctl->led_weir = t;
-    }
+    kahan_add(ctl->led_weir, ctl->led_weir_c, c * P.arbase);
+    // solute the previous step's melt brought into water (m: melt area of
+    // the cells that held water; meteoi and cmelt are still that step's)
+    if (P.meteo && P.ade)
+        for (int k = 0; k < P.nchem; k++)
+            kahan_add(ctl->sled[k], ctl->sled_c[k], ctl->meteoi * m * P.arbase * ctl->cmelt[k]);
     ctl->dt_raw = a;
     const double dt = fmin(ctl->print_next - ctl->tim, a);
     ctl->dtfl = dt;
@@ -689,6 +707,10 @@ __device__ inline void clock_forcing(const fx_params &P, fx_ctl *ctl, int mrow, 
             double dm = P.d32[ctl->cur][p];
             if (P.meteo) dm += ctl->meteoi * (P.melt_mult ? (double)P.melt_mult[p] : 1.0);
             if (dm <= 0.0) vin -= dm;
+            // its solute enters only water (common.cpp:350-361)
+            if (dm > 0.0 && P.ade)
+                for (int k = 0; k < P.nchem; k++)
+                    kahan_add(ctl->sled[k], ctl->sled_c[k], ctl->inflowi * P.arbase * ctl->cin[k]);
         }
         const double y = vin * P.arbase - ctl->led_melt_c;
         const double t = ctl->led_melt + y;
@@ -712,9 +734,9 @@ __device__ inline void clock_body(const fx_params &P, double *partials, int nblk
         if (threadIdx.x == 0) ctl->halt = 1;
         return;
     }
-    double a, b, c;
-    clock_fold(partials, 0, nblk, a, b, c);
-    clock_finish(P, ctl, a, b, c);
+    double a, b, c, m;
+    clock_fold(partials, 0, nblk, a, b, c, m);
+    clock_finish(P, ctl, a, b, c, m);
 }
 
 // one domain: the partials are folded by several blocks in fixed chunks into
@@ -730,13 +752,14 @@ __global__ __launch_bounds__(1024) void k32_clock(fx_params P, double *partials,
     }
     const int chunk = (nblk + (int)gridDim.x - 1) / (int)gridDim.x;
     const int lo = min(nblk, (int)blockIdx.x * chunk), hi = min(nblk, lo + chunk);
-    double a, b, c;
-    clock_fold(partials, lo, hi, a, b, c);
+    double a, b, c, m;
+    clock_fold(partials, lo, hi, a, b, c, m);
     __shared__ int last;
     if (threadIdx.x == 0) {
-        part2[3 * blockIdx.x] = a;
-        part2[3 * blockIdx.x + 1] = b;
-        part2[3 * blockIdx.x + 2] = c;
+        part2[FX32_NPV * blockIdx.x] = a;
+        part2[FX32_NPV * blockIdx.x + 1] = b;
+        part2[FX32_NPV * blockIdx.x + 2] = c;
+        part2[FX32_NPV * blockIdx.x + 3] = m;
         __threadfence();
         last = atomicAdd(ticket, 1u) == gridDim.x - 1;
     }
@@ -745,23 +768,26 @@ __global__ __launch_bounds__(1024) void k32_clock(fx_params P, double *partials,
     // second level, by the whole block: the other blocks' part2 entries are
     // read past this CU's cache (volatile), one entry per thread
     __threadfence();
-    double A = 9.e10, B = 0.0, Cw = 0.0;
+    double A = 9.e10, B = 0.0, Cw = 0.0, Mw = 0.0;
     for (int k = threadIdx.x; k < (int)gridDim.x; k += blockDim.x) {
-        const volatile double *q = part2 + 3 * (size_t)k;
+        const volatile double *q = part2 + FX32_NPV * (size_t)k;
         A = fmin(A, q[0]);
         B = fmax(B, q[1]);
         Cw += q[2];
+        Mw += q[3];
     }
-    __shared__ double sm2[3][16];
-    A = wmin(A); B = wmax(B); Cw = wsum(Cw);
+    __shared__ double sm2[4][16];
+    A = wmin(A); B = wmax(B); Cw = wsum(Cw); Mw = wsum(Mw);
     const int tid = threadIdx.x;
-    if ((tid & 63) == 0) { sm2[0][tid >> 6] = A; sm2[1][tid >> 6] = B; sm2[2][tid >> 6] = Cw; }
+    if ((tid & 63) == 0) { sm2[0][tid >> 6] = A; sm2[1][tid >> 6] = B; sm2[2][tid >> 6] = Cw; sm2[3][tid >> 6] = Mw; }
     __syncthreads();
     if (tid == 0) {
-        for (int w = 1; w < (int)(blockDim.x >> 6); w++) { A = fmin(A, sm2[0][w]); B = fmax(B, sm2[1][w]); Cw += sm2[2][w]; }
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+            A = fmin(A, sm2[0][w]); B = fmax(B, sm2[1][w]); Cw += sm2[2][w]; Mw += sm2[3][w];
+        }
         *ticket = 0;
     }
-    clock_finish(P, ctl, A, B, Cw);
+    clock_finish(P, ctl, A, B, Cw, Mw);
 }
 
 // batched: one block per member
@@ -948,8 +974,8 @@ int fx32_init_state(fx_domain d)
     t.npart = (int64_t)g.x * g.y;
     // room for the partials of the smallest column block fx_set_tiling allows
     const int64_t cap = (int64_t)g.x * ((P.ncols + FX32_CH_MIN - 1) / FX32_CH_MIN);
-    if ((rc = dalloc32(d, &t.partials, 3 * cap))) return rc;
-    if ((rc = dalloc32(d, &t.part2, 3 * FX32_FOLD_MAX))) return rc;
+    if ((rc = dalloc32(d, &t.partials, FX32_NPV * cap))) return rc;
+    if ((rc = dalloc32(d, &t.part2, FX32_NPV * FX32_FOLD_MAX))) return rc;
     if ((rc = dalloc32(d, &t.ticket, 1))) return rc;
     return FX_OK;
 }
@@ -1253,17 +1279,60 @@ int fx32_launch_volume(fx_domain d, double *vol_out)
 }
 
 // weir-face exchange of the last step, not yet folded by k32_clock
-int fx32_pending_weir(fx_domain d, double *vol)
+namespace {
+// solute mass per unit area (c h) summed over interior non-weir cells, block partials
+__global__ __launch_bounds__(256) void k32_solute(const float *mass, const uint8_t *flags, int64_t mm, double *part)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    double v = 0.0;
+    if (i < mm && (flags[i] & FXF_INNER) && !(flags[i] & FXF_WEIR)) v = (double)mass[i];
+    v = wsum(v);
+    __shared__ double sm[4];
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = sm[0] + sm[1] + sm[2] + sm[3];
+}
+} // namespace
+
+int fx32_launch_solute(fx_domain d, int k, double *mass_out)
+{
+    FX_HIP(hipMemcpyAsync(d->ctl_host, d->ctl, sizeof(fx_ctl), hipMemcpyDeviceToHost, d->stream));
+    FX_HIP(hipStreamSynchronize(d->stream));
+    const float *m = d->s32.mass[d->ctl_host->cur][k];
+    const size_t nb = (size_t)((d->ncell + 255) / 256);
+    double *part = nullptr;
+    FX_HIP(hipMalloc((void **)&part, nb * sizeof(double)));
+    hipLaunchKernelGGL(k32_solute, dim3((unsigned)nb), dim3(256), 0, d->stream, m, d->s32.flags, d->ncell, part);
+    std::vector<double> h(nb);
+    FX_HIP(hipMemcpyAsync(h.data(), part, nb * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+    FX_HIP(hipStreamSynchronize(d->stream));
+    FX_HIP(hipFree(part));
+    double s = 0.0, c = 0.0;
+    for (double v : h) { double y = v - c; double t = s + y; c = (t - s) - y; s = t; }
+    *mass_out = s * (double)d->desc.dxy * (double)d->desc.dxy;
+    return FX_OK;
+}
+
+int fx32_pending(fx_domain d, double *weir_vol, double *solute_area)
 {
     const size_t nb = (size_t)d->s32.npart;
-    std::vector<double> h(3 * nb);
-    FX_HIP(hipMemcpyAsync(h.data(), d->s32.partials, 3 * nb * sizeof(double), hipMemcpyDeviceToHost,
+    std::vector<double> h(FX32_NPV * nb);
+    FX_HIP(hipMemcpyAsync(h.data(), d->s32.partials, FX32_NPV * nb * sizeof(double), hipMemcpyDeviceToHost,
                           d->stream));
     FX_HIP(hipStreamSynchronize(d->stream));
-    double s = 0.0;
-    for (size_t i = 0; i < nb; i++) s += h[3 * i + 2];
-    *vol = s * (double)d->desc.dxy * (double)d->desc.dxy;
+    double w = 0.0, m = 0.0;
+    for (size_t i = 0; i < nb; i++) {
+        w += h[FX32_NPV * i + 2];
+        m += h[FX32_NPV * i + 3];
+    }
+    if (weir_vol) *weir_vol = w * (double)d->desc.dxy * (double)d->desc.dxy;
+    if (solute_area) *solute_area = m;
     return FX_OK;
+}
+
+int fx32_pending_weir(fx_domain d, double *vol)
+{
+    return fx32_pending(d, vol, nullptr);
 }
 
 // change the column-block lengths of the FP32 sweeps.  The pending weir-face
@@ -1272,18 +1341,25 @@ int fx32_pending_weir(fx_domain d, double *vol)
 int fx32_set_tiling(fx_domain d, int ch, int ach)
 {
     if (ch == d->ch && ach == d->ach) return FX_OK;
-    double pend = 0.0;
+    double pend = 0.0, area = 0.0;
     int rc;
-    if ((rc = fx32_pending_weir(d, &pend))) return rc;
+    if ((rc = fx32_pending(d, &pend, &area))) return rc;
     FX_HIP(hipMemcpyAsync(d->ctl_host, d->ctl, sizeof(fx_ctl), hipMemcpyDeviceToHost, d->stream));
     FX_HIP(hipStreamSynchronize(d->stream));
     fx_ctl *c = d->ctl_host;
-    const double y = pend - c->led_weir_c;
-    const double t = c->led_weir + y;
-    c->led_weir_c = (t - c->led_weir) - y;
-    c->led_weir = t;
+    // the partials are cleared: fold what they hold into the ledgers first
+    auto kahan = [](double &sum, double &comp, double v) {
+        const double y = v - comp;
+        const double t = sum + y;
+        comp = (t - sum) - y;
+        sum = t;
+    };
+    kahan(c->led_weir, c->led_weir_c, pend);
+    if (d->meteo && d->desc.ade_solver)
+        for (int k = 0; k < d->desc.nchem; k++)
+            kahan(c->sled[k], c->sled_c[k], c->meteoi * area * (double)d->desc.dxy * d->desc.dxy * c->cmelt[k]);
     FX_HIP(hipMemcpyAsync(d->ctl, c, sizeof(fx_ctl), hipMemcpyHostToDevice, d->stream));
-    FX_HIP(hipMemsetAsync(d->s32.partials, 0, 3 * (size_t)d->s32.npart * sizeof(double), d->stream));
+    FX_HIP(hipMemsetAsync(d->s32.partials, 0, FX32_NPV * (size_t)d->s32.npart * sizeof(double), d->stream));
     d->ch = ch;
     d->ach = ach;
     fx_params P = fx_make_params(d);

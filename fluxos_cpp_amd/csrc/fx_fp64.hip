@@ -602,6 +602,43 @@ int fx_launch_step_end(fx_domain d)
     return fx_hip_check(hipGetLastError(), "k_step_end");
 }
 
+namespace {
+// solute mass per unit area (c h) of interior non-weir cells, block partials
+__global__ __launch_bounds__(256) void k64_solute(fx_params P, fx_state64 s, int k, double *part)
+{
+    int irow, icol;
+    double v = 0.0;
+    if (fx_interior(P, irow, icol)) {
+        const size_t p = (size_t)irow + (size_t)icol * P.ld;
+        if (!s.weir[p]) v = s.conc[k][p] * s.h[p];
+    }
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    __shared__ double sm[4];
+    const int tid = threadIdx.x + threadIdx.y * BX;
+    if ((tid & 63) == 0) sm[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0) part[blockIdx.x + blockIdx.y * gridDim.x] = sm[0] + sm[1] + sm[2] + sm[3];
+}
+} // namespace
+
+int fx64_launch_solute(fx_domain d, int k, double *mass_out)
+{
+    fx_params P = fx_make_params(d);
+    dim3 g = grid_of(P);
+    const size_t nb = (size_t)g.x * g.y;
+    double *part = nullptr;
+    FX_HIP(hipMallocAsync((void **)&part, nb * sizeof(double), d->stream));
+    hipLaunchKernelGGL(k64_solute, g, dim3(BX, BY), 0, d->stream, P, d->s64, k, part);
+    std::vector<double> h(nb);
+    FX_HIP(hipMemcpyAsync(h.data(), part, nb * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+    FX_HIP(hipFreeAsync(part, d->stream));
+    FX_HIP(hipStreamSynchronize(d->stream));
+    double s = 0.0, c = 0.0;
+    for (double v : h) { double y = v - c; double t = s + y; c = (t - s) - y; s = t; }
+    *mass_out = s * P.arbase;
+    return FX_OK;
+}
+
 int fx64_launch_volume(fx_domain d, double *vol_out)
 {
     fx_params P = fx_make_params(d);

@@ -26,6 +26,9 @@ struct fx_ctl {
     // net volume entering non-weir cells through faces shared with weir cells
     double led_melt, led_melt_c;      // Kahan sum and compensation
     double led_weir, led_weir_c;
+    // FP64 solute ledger (extension): solute the melt and the inflow brought
+    // into water, per solute (Kahan sum and compensation)
+    double sled[FX_MAX_CHEM], sled_c[FX_MAX_CHEM];
     int32_t it;          // solver steps so far
     int32_t gate;        // hpall != 0 for the current step
     int32_t halt;        // steps are no-ops while set
@@ -164,3 +167,5 @@ int fx32_set_field(fx_domain d, int field, const void *host);
 int fx32_launch_volume(fx_domain d, double *vol_out);
 int fx32_launch_ade(fx_domain d);
 int fx32_pending_weir(fx_domain d, double *vol);
+int fx32_launch_solute(fx_domain d, int k, double *mass_out);
+int fx64_launch_solute(fx_domain d, int k, double *mass_out);

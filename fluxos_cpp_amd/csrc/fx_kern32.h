@@ -36,6 +36,7 @@ struct a32p {
 #define FX32_AOWN 58          // rows owned per ADE-sweep strip
 #define FX32_ACH 64           // default columns per ADE-sweep block
 #define FX32_FOLD_MAX 256     // blocks of the clock's partial fold
+#define FX32_NPV 4            // values per step-sweep partial: min dt, max h, weir volume, solute melt area
 
 // One member of a batch, in device memory: everything its kernels need.
 struct fx_bslot {
@@ -62,6 +63,9 @@ int fx32_npart(fx_domain d);
 int fx32_ade_gx(fx_domain d);
 int fx32_ade_blocks(fx_domain d);
 int fx32_set_tiling(fx_domain d, int ch, int ach);
+// sums over the step-sweep partials not yet folded by the clock: weir-face
+// volume [m3] and the melt area that brought solute [cells, times the multiplier]
+int fx32_pending(fx_domain d, double *weir_vol, double *solute_area);
 
 // batched launches over nslot members; tmap / amap give, per block, the
 // member (high 12 bits) and the member's own block index (low 20 bits)

@@ -171,6 +171,14 @@ int fx_advance(fx_domain dom, double t_end, double print_next, int32_t max_steps
  * inflow cell's deficit filled up to the bed, common.cpp:324-325),
  * out[2]=net weir inflow.  The ledger is kept by FP32-mixed domains. */
 int fx_mass_balance(fx_domain dom, double out[3]);
+/* Solute ichem's mass in the non-weir cells, sum of c h dxy^2 in FP64
+ * (out[0]), and the running FP64 ledger of the solute the melt and the
+ * inflow brought into water (out[1], FP32-mixed domains; 0 for FP64
+ * domains) (extension; no counterpart in the reference).  The reference's
+ * scheme does not conserve solute: it is zeroed in cells that dry and in weir
+ * cells, and the ADE's min/max clamp adds or removes it, so out[0] and
+ * out[1] differ by those sinks.                                            */
+int fx_solute_balance(fx_domain dom, int32_t ichem, double out[2]);
 /* Tile shape of the FP32-mixed sweeps (extension; no counterpart in the
  * reference): columns walked per block by the step sweep (ch) and by the
  * ADE sweep (ach), 8..4096; defaults 32 and 64.  Results do not depend on it.

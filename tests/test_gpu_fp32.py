@@ -193,3 +193,57 @@ def test_inflow_on_nodata_bed_is_an_error():
         with pytest.raises(capi.FxError, match="NODATA"):
             d.advance(3600.0, 600.0)
         d.close()
+
+
+def test_fp32_solute_ledger_closed_form():
+    """A basin under 5 cm of water everywhere receives melt at R mm/day with
+    2 mg/L: every cell holds water before each melt, so the ledger of solute
+    brought in is R/86.4e6 * t * (cells) * dxy^2 * 2, whatever the dt
+    sequence.  The solute kept in the non-weir cells is at most that, and at
+    least their share of it (weir cells zero their solute)."""
+    n, H, rate, conc, dxy = 64, 0.05, 40.0, 2.0, 2
+    dem = np.full((n, n), 500.0, dtype=np.float32)
+    dem[0, :] = dem[-1, :] = dem[:, 0] = dem[:, -1] = -99999.0
+    zb, ks, weir = capi.read_geo(dem, -99999.0, 1e-3)
+    zb, z, h, ldry, c0 = capi.initiation(zb, True)
+    z[1:-1, 1:-1] = zb[1:-1, 1:-1] + H
+    h[1:-1, 1:-1] = H
+    ldry[1:-1, 1:-1] = 0.0
+    d = capi.HipDomain(n, n, dxy, 1e-3, 1, (0.01,), True, capi.FP32_MIXED)
+    d.set_static(zb, ks, weir)
+    d.set("z", z)
+    d.set("h", h)
+    d.set("ldry", ldry)
+    d.set_conc(0, c0)
+    d.set_meteo(gen.meteo_table(np.array([0.0, 1e6]), rate, [conc]))
+    d.set_clock(0.0, 0.0, 0)
+    info = d.advance(1e9, 1e9, 40)
+    mass, added = d.solute_balance(0)
+    # cells that hold water: all but the corner whose four bed vertices are
+    # NODATA (its bed is 0/0, initiate.cpp:85)
+    wet = int(np.isfinite(zb[1:-1, 1:-1]).sum())
+    expect = rate / 86.4e6 * info.tim * wet * dxy * dxy * conc
+    share = float((weir[1:-1, 1:-1] == 0).sum()) / wet
+    print(f"t={info.tim:.3f} ledger {added:.9e} closed form {expect:.9e} mass {mass:.9e} "
+          f"(non-weir share {share:.4f})", flush=True)
+    assert abs(added - expect) <= 1e-12 * expect
+    assert share * 0.95 * added <= mass <= added * (1 + 1e-6)
+    d.close()
+
+
+def test_fp32_solute_mass_against_fp64():
+    """Solute held in the basin after a span: FP32-mixed with the fast ADE
+    against the FP64 path with the reference's exact sweep.  Most of the
+    solute the melt brings in is destroyed (melt into water that then dries,
+    weir cells), so what is left is a small residual; the two paths' residuals
+    are compared against the solute brought in (1%)."""
+    sc = gen.scenario_c1(64, seed=1)
+    _, d64, d32 = _pair(sc, ade=True)
+    for d in (d64, d32):
+        _advance_to(d, 1200.0)
+    m64, m32 = d64.solute_balance(0)[0], d32.solute_balance(0)[0]
+    added = d32.solute_balance(0)[1]
+    print(f"solute mass fp64 {m64:.9e} fp32 {m32:.9e} rel {abs(m32 - m64) / m64:.3e}; fp32 ledger in {added:.9e}",
+          flush=True)
+    assert m64 > 0 and abs(m32 - m64) <= 1e-2 * added
+    assert m32 <= added * (1 + 1e-6)
