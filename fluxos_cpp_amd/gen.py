@@ -81,7 +81,10 @@ def diamond_square(n: int, roughness: float, rng: np.random.Generator) -> np.nda
     """Fractal surface on a (2^k+1)^2 lattice covering n x n, scaled to [0, 1].
 
     Midpoint displacement: the displacement amplitude shrinks by 2^-roughness
-    per level (roughness plays the role of the Hurst exponent H)."""
+    per level (roughness plays the role of the Hurst exponent H).  Each square
+    step averages the in-range ones of its four neighbours (up, down, left,
+    right, summed in that order) and takes its random displacements in
+    row-major order of the points it sets; only those points are touched."""
     k = max(1, math.ceil(math.log2(max(n - 1, 1))))
     m = 2 ** k + 1
     a = np.zeros((m, m), dtype=np.float64)
@@ -95,26 +98,29 @@ def diamond_square(n: int, roughness: float, rng: np.random.Generator) -> np.nda
         c = 0.25 * (a[0:-1:step, 0:-1:step] + a[0:-1:step, step::step]
                     + a[step::step, 0:-1:step] + a[step::step, step::step])
         a[h::step, h::step] = c + amp * rng.uniform(-1, 1, c.shape)
-        # square: edge midpoints (rows at multiples of step, cols at odd h)
-        s = np.zeros_like(a)
-        cnt = np.zeros_like(a)
-        for di, dj in ((-h, 0), (h, 0), (0, -h), (0, h)):
-            src = np.roll(np.roll(a, -di, axis=0), -dj, axis=1)
-            valid = np.ones_like(a, dtype=bool)
-            if di < 0:
-                valid[:h, :] = False
-            if di > 0:
-                valid[-h:, :] = False
-            if dj < 0:
-                valid[:, :h] = False
-            if dj > 0:
-                valid[:, -h:] = False
-            s += np.where(valid, src, 0.0)
-            cnt += valid
-        mask = np.zeros_like(a, dtype=bool)
-        mask[0::step, h::step] = True
-        mask[h::step, 0::step] = True
-        a[mask] = s[mask] / cnt[mask] + amp * rng.uniform(-1, 1, int(mask.sum()))
+        # square: edge midpoints.  Type A: rows 0::step, cols h::step; type B:
+        # rows h::step, cols 0::step.  Row-major order interleaves them by row.
+        ra, ca = np.arange(0, m, step), np.arange(h, m, step)
+        rb, cb = np.arange(h, m, step), np.arange(0, m, step)
+
+        def average(rows, cols):
+            s = np.zeros((len(rows), len(cols)))
+            cnt = np.zeros_like(s)
+            R, Cc = np.meshgrid(rows, cols, indexing="ij")
+            for di, dj in ((-h, 0), (h, 0), (0, -h), (0, h)):
+                ii, jj = R + di, Cc + dj
+                ok = (ii >= 0) & (ii < m) & (jj >= 0) & (jj < m)
+                v = np.where(ok, a[np.clip(ii, 0, m - 1), np.clip(jj, 0, m - 1)], 0.0)
+                s += v
+                cnt += ok
+            return s / cnt
+
+        va, vb = average(ra, ca), average(rb, cb)
+        na, nb = len(ca), len(cb)
+        r = rng.uniform(-1, 1, len(ra) * na + len(rb) * nb)
+        r = np.concatenate([r, np.zeros(nb)]).reshape(len(ra), na + nb)
+        a[0::step, h::step] = va + amp * r[:, :na]
+        a[h::step, 0::step] = vb + amp * r[:-1, na:]
         step = h
         amp *= scale
     a = a[:n, :n]
