@@ -102,20 +102,27 @@ def diamond_square(n: int, roughness: float, rng: np.random.Generator) -> np.nda
         # rows h::step, cols 0::step.  Row-major order interleaves them by row.
         ra, ca = np.arange(0, m, step), np.arange(h, m, step)
         rb, cb = np.arange(h, m, step), np.arange(0, m, step)
-
-        def average(rows, cols):
-            s = np.zeros((len(rows), len(cols)))
-            cnt = np.zeros_like(s)
-            R, Cc = np.meshgrid(rows, cols, indexing="ij")
-            for di, dj in ((-h, 0), (h, 0), (0, -h), (0, h)):
-                ii, jj = R + di, Cc + dj
-                ok = (ii >= 0) & (ii < m) & (jj >= 0) & (jj < m)
-                v = np.where(ok, a[np.clip(ii, 0, m - 1), np.clip(jj, 0, m - 1)], 0.0)
-                s += v
-                cnt += ok
-            return s / cnt
-
-        va, vb = average(ra, ca), average(rb, cb)
+        centre = a[h::step, h::step]                  # set by this level's diamond step
+        # type A: up (rows after the first), down (rows before the last), left, right
+        sa = np.zeros((len(ra), len(ca)))
+        na_ = np.zeros_like(sa)
+        sa[1:] += centre
+        na_[1:] += 1
+        sa[:-1] += centre
+        na_[:-1] += 1
+        sa += a[0::step, 0:m - 1:step]
+        sa += a[0::step, step::step]
+        na_ += 2
+        # type B: up, down, left (columns after the first), right (before the last)
+        sb = np.zeros((len(rb), len(cb)))
+        nb_ = np.full_like(sb, 2.0)
+        sb += a[0:m - 1:step, 0::step]
+        sb += a[step::step, 0::step]
+        sb[:, 1:] += centre
+        nb_[:, 1:] += 1
+        sb[:, :-1] += centre
+        nb_[:, :-1] += 1
+        va, vb = sa / na_, sb / nb_
         na, nb = len(ca), len(cb)
         r = rng.uniform(-1, 1, len(ra) * na + len(rb) * nb)
         r = np.concatenate([r, np.zeros(nb)]).reshape(len(ra), na + nb)
