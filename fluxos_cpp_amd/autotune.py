@@ -7,7 +7,7 @@ against the number of waves in flight.  Results do not depend on it (every
 cell is computed by one lane from the same inputs either way).
 
 ``tune(n)`` times every candidate on a synthetic basin of side n (the
-fractal-DEM generator, two solutes) spun up on the device to t = 3600 s and
+fractal-DEM generator, two solutes) spun up on the device to t = 7200 s and
 then run on, candidate after candidate, and keeps the fastest for each kernel; ``main`` tunes a ladder of
 sizes and writes ``tuning.json`` beside this file, which
 ``HipDomain.from_scenario`` consults (nearest size by cell count).
@@ -25,8 +25,8 @@ from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 TABLE = PKG / "tuning.json"
-CANDIDATES_CH = (8, 16, 32, 64, 128)
-CANDIDATES_ACH = (16, 32, 64, 128, 256)
+CANDIDATES_CH = (8, 12, 16, 24, 32, 64)
+CANDIDATES_ACH = (16, 32, 48, 64, 96, 128)
 FX32_ACH_DEFAULT = 64
 
 
@@ -44,7 +44,7 @@ def _kernel_us(dom, steps: int) -> dict[str, float]:
     return out
 
 
-def tune(n: int, steps: int = 40, t_wet: float = 3600.0) -> dict:
+def tune(n: int, steps: int = 40, t_wet: float = 7200.0) -> dict:
     from . import capi, gen
     sc = gen.scenario_c2(n)
     init = capi.prepare(sc)
