@@ -63,7 +63,7 @@ struct actx {
     double meteoi, inflowi;
     uint32_t inflow_p;        // linear index of the inflow cell, ~0u = none
     float cm[ANC], cin[ANC], kdiff[ANC];
-    float hdry, dx, area_dt, dt, area;
+    float hdry, dx, area_dt, dt_area;
     int row, mcols;
     uint32_t ld;
     bool row_ok, transport;
@@ -114,34 +114,35 @@ __device__ inline void a_derive(const actx &X, const apref<NC> &a, ageo &g, floa
     g.w = w;
     g.fe = inner ? a.fe : 0.f;
     g.fn = inner ? a.fn : 0.f;
+    // one reciprocal per depth, shared by the solutes (h >= hdry > 0 where wet)
+    const float ih0 = h0 > 0.f ? __builtin_amdgcn_rcpf(h0) : 0.f;
+    const float ihn = (w && wp) ? __builtin_amdgcn_rcpf(g.h) : 0.f;
 #pragma unroll
     for (int k = 0; k < NC; k++) {
         const float mv = inner ? a.m[k] + mlt * X.cm[k] + inf * X.cin[k] : 0.f;
-        float c = 0.f, r = 0.f;
-        if (h0 > 0.f || w) {
-            r = h0 > 0.f ? mv / h0 : 0.f;
-            if (w) c = wp ? mv / g.h : r;
-        }
+        const float r = mv * ih0;
+        const float c = w ? (wp ? mv * ihn : r) : 0.f;
         c1[k] = X.transport ? c : mv;
         cr[k] = r;
     }
 }
 
-// face flux before the outflow limiter, from the cell P (value cp, depth hp)
-// towards its neighbour Q (cq, hq); cu is the value one further upstream of P,
-// cd one further downstream of Q, wu/wd their wet flags, q the face discharge
-__device__ inline float a_face(float q, float cp, float hp, float cq, float hq, bool wq, float cu, bool wu,
-                               float cd, bool wd, float kdiff, float dx, float area_dt)
+// face flux of one solute before the outflow limiter, from the cell P (value
+// cp) towards its neighbour Q (cq, wet flag wq); cu is the value one further
+// upstream of P, cd one further downstream of Q, wu/wd their wet flags, q the
+// face discharge; sq = sqrt(max(hp*hq, 1e-4)) and cap = -hq*area/dt are the
+// solute-independent parts (ADEsolver_calc.cpp:149-241)
+__device__ inline float a_face(float q, float cp, float cq, bool wq, float cu, bool wu, float cd, bool wd,
+                               float sq, float cap, float kdiff, float dx)
 {
     if (!wq) return 0.f;                          // dry neighbour: fem = 0, no diffusion
-    const float hne = sqrtf(fmaxf(hp * hq, 1e-4f)) * kdiff;
-    const float pd = -hne * (cq - cp);
+    const float pd = -(sq * kdiff) * (cq - cp);
     float fm;
     if (q > 0.f) fm = wu ? -.125f * cu + .75f * cp + .375f * cq : .5f * (cp + cq);
     else fm = wd ? .375f * cp + .75f * cq - .125f * cd : .5f * (cp + cq);
     fm = fmaxf(0.f, fm);
     float pf = q * fm * dx + pd;
-    if (pf < 0.f) pf = fmaxf(pf, -(cq * hq) * area_dt);
+    if (pf < 0.f) pf = fmaxf(pf, cq * cap);
     return pf;
 }
 
@@ -151,7 +152,7 @@ __device__ inline void a_limit(float cv, float &pfe, float &qfn)
     if (cv > 0.f) {
         if (pfe > 0.f && qfn > 0.f) {
             if (pfe + qfn > cv) {
-                const float cf = qfn / (pfe + qfn);
+                const float cf = qfn * __builtin_amdgcn_rcpf(pfe + qfn);
                 pfe = (1.f - cf) * cv;
                 qfn = cf * cv;
             }
@@ -208,22 +209,32 @@ __device__ inline void a_column(const actx &X, awin<NC> &W, int c, bool upd, boo
     const bool wEE = __shfl_down((int)gP.w, 2, 64);
     const float hE = dn1(gP.h);
     const uint32_t p = (uint32_t)X.row + (uint32_t)c * X.ld;
+    const bool act = X.transport && gP.w;
+    // per cell, shared by the solutes: the faces' eddy terms and inflow
+    // floors, h*area/dt of the limiter and dt/(area*h) of the update
+    float sE = 0.f, sN = 0.f, hA = 0.f, kup = 0.f;
+    if (act) {
+        sE = sqrtf(fmaxf(gP.h * hE, 1e-4f));
+        sN = sqrtf(fmaxf(gP.h * gN.h, 1e-4f));
+        hA = gP.h * X.area_dt;
+        kup = X.dt_area * __builtin_amdgcn_rcpf(gP.h);
+    }
 #pragma unroll
     for (int k = 0; k < NC; k++) {
         const float cp = W.c1[1][k];
         float pe = 0.f, pn = 0.f;
         const float cW = up1(cp), cE = dn1(cp), cEE = dn2(cp);
         const float crE = dn1(W.cr[1][k]);
-        if (X.transport && gP.w) {
-            pe = a_face(gP.fe, cp, gP.h, cE, hE, wE, cW, wW, cEE, wEE, X.kdiff[k], X.dx, X.area_dt);
-            pn = a_face(gP.fn, cp, gP.h, W.c1[2][k], gN.h, gN.w, W.c1[0][k], gS.w, W.c1[3][k], gNN.w,
-                        X.kdiff[k], X.dx, X.area_dt);
+        if (act) {
+            pe = a_face(gP.fe, cp, cE, wE, cW, wW, cEE, wEE, sE, -hE * X.area_dt, X.kdiff[k], X.dx);
+            pn = a_face(gP.fn, cp, W.c1[2][k], gN.w, W.c1[0][k], gS.w, W.c1[3][k], gNN.w, sN,
+                        -gN.h * X.area_dt, X.kdiff[k], X.dx);
         }
         // limiting of this cell: inflow terms of its W and S faces before their
         // own cells' limiting
         const float pe_w = up1(pe);
         float e = pe, n = pn;
-        if (gP.w) a_limit(cp * gP.h * X.area_dt + pe_w + W.pn_s[k], e, n);
+        if (gP.w) a_limit(cp * hA + pe_w + W.pn_s[k], e, n);
         else { e = 0.f; n = 0.f; }
         const float e_w = up1(e);
         if (upd && own) {
@@ -233,8 +244,7 @@ __device__ inline void a_column(const actx &X, awin<NC> &W, int c, bool upd, boo
             } else if (!gP.w) {
                 out = 0.f;                               // dry after the update (:69-71,119-124)
             } else {
-                const float dc = (e_w - e + W.n_s[k] - n) * X.dt / X.area;
-                float con = cp + dc / gP.h;
+                float con = cp + (e_w - e + W.n_s[k] - n) * kup;
                 // bounds of pass 1: W and S already rescaled, E and N not yet (:51-62)
                 const float cs = W.c1[0][k], cn = W.cr[2][k];
                 con = fminf(fmaxf(fmaxf(cW, crE), fmaxf(cs, cn)), con);
@@ -267,8 +277,7 @@ __device__ inline void ade_tile(const fx_params &P, const a32p &A, fx_ctl *ctl, 
     X.hdry = (float)P.hdry;
     X.dx = (float)P.dxy;
     X.area_dt = (float)P.arbase / dt;
-    X.dt = dt;
-    X.area = (float)P.arbase;
+    X.dt_area = dt / (float)P.arbase;
 #pragma unroll
     for (int k = 0; k < NC; k++) {
         X.m_in[k] = A.mass[cur][A.k0 + k];
