@@ -31,7 +31,7 @@
 // masses read and written.
 #include "fx_kern32.h"
 
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
 
 #define AOWN FX32_AOWN        // rows owned per strip
 #define ANC 4                 // solutes per launch

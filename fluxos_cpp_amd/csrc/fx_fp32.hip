@@ -278,13 +278,26 @@ struct pref {                 // one cell's HBM values in flight
 };
 #define PF_INFLOW 0x100u      // the cell takes the inflow point source
 
+// Three columns (c-1, c, c+1) of the values the flux stencil reads; the bed
+// and the FP64 depth only for c and c+1, and the fluxes only where the update
+// reads them: the E face of column c and the N faces of columns c-1 and c.
+// The two-column arrays are indexed by column parity.  9344 bytes, so four
+// waves per SIMD fit in the LDS.
 struct ring {
-    float dm[3][NS], qx[3][NS], qy[3][NS], u[3][NS], v[3][NS], sq[3][NS], zbh[3][NS], zbl[3][NS];
+    float dm[3][NS], qx[3][NS], qy[3][NS], u[3][NS], v[3][NS], sq[3][NS];
+    float zbh[2][NS], zbl[2][NS];
     uint8_t fl[3][NS];        // LW | LX | LI | LS
     uint8_t raw[3][NS];       // flag bytes
-    double dm64[3][NS];       // FP64 depth after melt
-    float f[2][6][NS];        // fluxes
+    double dm64[2][NS];       // FP64 depth after melt
+    float fe[3][NS];          // fe_1..fe_3 of column c
+    float fn[2][3][NS];       // fn_1..fn_3 of columns c-1 and c
 };
+
+// flux m (0..5 = fe_1..fe_3, fn_1..fn_3) of row i of the column of parity par
+__device__ inline float &rflux(ring &R, int par, int m, int i)
+{
+    return m < 3 ? R.fe[m][i] : R.fn[par][m - 3][i];
+}
 
 // per-block constants of the sweep
 struct sweep_ctx {
@@ -322,8 +335,9 @@ __device__ inline void load_col(const sweep_ctx &X, int c, pref &a)
 // L: this lane's cell of the column into ring slot SL (melt of this step,
 // hydro loop 1, cell velocities)
 template <int SL>
-__device__ inline void store_col(const sweep_ctx &X, ring &R, const pref &a, bool own_col, double &ms)
+__device__ inline void store_col(const sweep_ctx &X, ring &R, const pref &a, int col, bool own_col, double &ms)
 {
+    const int par = col & 1;
     const int i = X.k;
     const bool inner = a.f & FXF_INNER;
     // melt area that brings solute (cells holding water before the melt,
@@ -347,12 +361,12 @@ __device__ inline void store_col(const sweep_ctx &X, ring &R, const pref &a, boo
     R.u[SL][i] = qx * den;
     R.v[SL][i] = qy * den;
     R.sq[SL][i] = sqrtf(h);
-    R.zbh[SL][i] = inner ? a.zbh : 0.f;
-    R.zbl[SL][i] = inner ? a.zbl : 0.f;
+    R.zbh[par][i] = inner ? a.zbh : 0.f;
+    R.zbl[par][i] = inner ? a.zbl : 0.f;
     R.fl[SL][i] = (uint8_t)((wet ? LW : 0) | ((a.f & FXF_WEIR) ? LX : 0) | (inner ? LI : 0) |
                             ((a.f & FXF_STALE) ? LS : 0));
     R.raw[SL][i] = (uint8_t)a.f;
-    R.dm64[SL][i] = d;
+    R.dm64[par][i] = d;
 }
 
 // F(c): fluxes of this lane's row; column c in slot S0, c-1 in SS, c+1 in SN
@@ -364,8 +378,9 @@ __device__ inline void flux_col(const sweep_ctx &X, ring &R, int c)
     const uint8_t fl = R.fl[S0][i];
     if (X.gate && X.flux_lane && (fl & LI) && !(fl & LX)) {
         const bool wetE = R.fl[S0][i + 1] & LW, wetN = R.fl[SN][i] & LW;
-        const float dzbE = (R.zbh[S0][i + 1] - R.zbh[S0][i]) + (R.zbl[S0][i + 1] - R.zbl[S0][i]);
-        const float dzbN = (R.zbh[SN][i] - R.zbh[S0][i]) + (R.zbl[SN][i] - R.zbl[S0][i]);
+        const int pc = c & 1, pn = pc ^ 1;
+        const float dzbE = (R.zbh[pc][i + 1] - R.zbh[pc][i]) + (R.zbl[pc][i + 1] - R.zbl[pc][i]);
+        const float dzbN = (R.zbh[pn][i] - R.zbh[pc][i]) + (R.zbl[pn][i] - R.zbl[pc][i]);
         if (!(fl & LW)) {
             const bool wetW = R.fl[S0][i - 1] & LW, wetS = R.fl[SS][i] & LW;
             if (!wetW && !wetE && !wetS && !wetN) {
@@ -395,7 +410,7 @@ __device__ inline void flux_col(const sweep_ctx &X, ring &R, int c)
         }
     }
     const int fs = c & 1;
-    for (int m = 0; m < 6; m++) R.f[fs][m][i] = f[m];
+    for (int m = 0; m < 6; m++) rflux(R, fs, m, i) = f[m];
 }
 
 // U(c): hydro loops 3 and 4 for this lane's own cell, and its CFL bound
@@ -411,7 +426,7 @@ __device__ inline void update_col(const sweep_ctx &X, ring &R, int c, double &dt
     const uint8_t f = R.raw[S0][i];
     const uint8_t fl = R.fl[S0][i];
     const bool weir = f & FXF_WEIR;
-    double d = R.dm64[S0][i];
+    double d = R.dm64[c & 1][i];
     float qxn, qyn;
     uint8_t nf = f;
     float fe1P = 0.f, fn1P = 0.f;
@@ -420,15 +435,15 @@ __device__ inline void update_col(const sweep_ctx &X, ring &R, int c, double &dt
         qyn = R.qy[S0][i];
     } else {
         const float kf = X.kf, dtf = X.dtf;
-        const float fe1W = R.f[fs][0][i - 1], fn1S = R.f[fsS][3][i];
-        fe1P = R.f[fs][0][i];
-        fn1P = R.f[fs][3][i];
+        const float fe1W = rflux(R, fs, 0, i - 1), fn1S = rflux(R, fsS, 3, i);
+        fe1P = rflux(R, fs, 0, i);
+        fn1P = rflux(R, fs, 3, i);
         float dqx = 0.f, dqy = 0.f;
         if (!weir) {
             const float a1 = fe1W * kf, b1 = fe1P * kf, c1 = fn1S * kf, e1 = fn1P * kf;
             d += ((double)a1 - (double)b1) + ((double)c1 - (double)e1);
-            dqx = ((R.f[fs][1][i - 1] - R.f[fs][1][i]) + (R.f[fsS][4][i] - R.f[fs][4][i])) * kf;
-            dqy = ((R.f[fs][2][i - 1] - R.f[fs][2][i]) + (R.f[fsS][5][i] - R.f[fs][5][i])) * kf;
+            dqx = ((rflux(R, fs, 1, i - 1) - rflux(R, fs, 1, i)) + (rflux(R, fsS, 4, i) - rflux(R, fs, 4, i))) * kf;
+            dqy = ((rflux(R, fs, 2, i - 1) - rflux(R, fs, 2, i)) + (rflux(R, fsS, 5, i) - rflux(R, fs, 5, i))) * kf;
             if (R.fl[S0][i + 1] & LX) wv -= (double)b1;        // into / out of a weir cell
             if (R.fl[SN][i] & LX) wv -= (double)e1;
         }
@@ -447,9 +462,9 @@ __device__ inline void update_col(const sweep_ctx &X, ring &R, int c, double &dt
                                 !(R.fl[SS][i] & LW) && !(R.fl[SN][i] & LW);
             if (!alldry) {
                 bool nz = false;
-                for (int m = 0; m < 6; m++) nz |= R.f[fs][m][i] != 0.f;
+                for (int m = 0; m < 6; m++) nz |= rflux(R, fs, m, i) != 0.f;
                 if (nz) {
-                    for (int m = 0; m < 6; m++) S.f6[(uint32_t)m * (uint32_t)S.mm + p] = R.f[fs][m][i];
+                    for (int m = 0; m < 6; m++) S.f6[(uint32_t)m * (uint32_t)S.mm + p] = rflux(R, fs, m, i);
                     nf |= FXF_STALE;
                 } else {
                     nf &= ~FXF_STALE;
@@ -485,7 +500,7 @@ __device__ inline void column_step(const sweep_ctx &X, ring &R, int c, int c_las
 {
     constexpr int SN = (SC + 1) % 3, SS = (SC + 2) % 3;
     __syncthreads();
-    store_col<SN>(X, R, a, c + 1 <= c_last, ms);      // column c+1 (prefetched last step)
+    store_col<SN>(X, R, a, c + 1, c + 1 <= c_last, ms);   // column c+1 (prefetched last step)
     if (c + 2 <= c_last + 1) load_col(X, c + 2, a);
     __syncthreads();
     flux_col<SC, SS, SN>(X, R, c);
@@ -541,9 +556,9 @@ __device__ inline void sweep_tile(const fx_params &P, const s32p &S, fx_ctl *ctl
     pref a;
     double dtm = 9.e10, hmx = 0.0, wv = 0.0, ms = 0.0;
     // prologue: columns c0-2 (slot 1), c0-1 (slot 2), c0 (slot 0), fluxes of c0-1
-    load_col(X, c0 - 2, a); store_col<1>(X, R, a, false, ms);
-    load_col(X, c0 - 1, a); store_col<2>(X, R, a, false, ms);
-    load_col(X, c0, a); store_col<0>(X, R, a, true, ms);
+    load_col(X, c0 - 2, a); store_col<1>(X, R, a, c0 - 2, false, ms);
+    load_col(X, c0 - 1, a); store_col<2>(X, R, a, c0 - 1, false, ms);
+    load_col(X, c0, a); store_col<0>(X, R, a, c0, true, ms);
     load_col(X, c0 + 1, a);
     __syncthreads();
     flux_col<2, 1, 0>(X, R, c0 - 1);
