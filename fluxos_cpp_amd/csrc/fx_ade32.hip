@@ -31,6 +31,10 @@
 // masses read and written.
 #include "fx_kern32.h"
 
+// multiply-adds are fused within one source expression only: the column
+// routines are inlined several times (block prologue, loop body, tail), and
+// fusing across statements can differ between copies, which would make a
+// cell's rounding depend on its place in the tile and results on the tiling
 #pragma clang fp contract(on)
 
 #define AOWN FX32_AOWN        // rows owned per strip

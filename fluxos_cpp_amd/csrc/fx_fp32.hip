@@ -29,7 +29,11 @@
 
 #include <cstdlib>
 
-#pragma clang fp contract(fast)
+// multiply-adds are fused within one source expression only: the column
+// routines are inlined several times (block prologue, loop body, tail), and
+// fusing across statements can differ between copies, which would make a
+// cell's rounding depend on its place in the tile and results on the tiling
+#pragma clang fp contract(on)
 
 
 namespace {
@@ -96,7 +100,7 @@ __device__ inline void solver_dry32(const consts32 &K, float dp, float de, float
 // term (:191-200) are per-cell quantities, computed once when a cell is
 // loaded (u, v below); likewise sqrt(h) for the eddy coefficients (:188-189).
 struct nb32 {
-    float dp, de, dn, dw, ds;          // depths after melt (may be negative)
+    float dp, de, dn;                  // depths after melt (may be negative)
     float dzbE, dzbN;                  // bed differences E-P, N-P
     float qp, qe, qn;                  // qx at P, E, N
     float rp, re, rn;                  // qy at P, E, N
@@ -278,16 +282,17 @@ struct pref {                 // one cell's HBM values in flight
 };
 #define PF_INFLOW 0x100u      // the cell takes the inflow point source
 
-// Three columns (c-1, c, c+1) of the values the flux stencil reads; the bed
-// and the FP64 depth only for c and c+1, and the fluxes only where the update
-// reads them: the E face of column c and the N faces of columns c-1 and c.
-// The two-column arrays are indexed by column parity.  9344 bytes, so four
-// waves per SIMD fit in the LDS.
+// Three columns (c-1, c, c+1) of what the flux stencil reads from the S
+// neighbour (u and the wet flags); columns c and c+1 of the rest, and the
+// fluxes only where the update reads them: the E face of column c and the N
+// faces of columns c-1 and c.  The two-column arrays are indexed by column
+// parity.  8000 bytes per wave.
 struct ring {
-    float dm[3][NS], qx[3][NS], qy[3][NS], u[3][NS], v[3][NS], sq[3][NS];
+    float u[3][NS];
+    float dm[2][NS], qx[2][NS], qy[2][NS], v[2][NS], sq[2][NS];
     float zbh[2][NS], zbl[2][NS];
     uint8_t fl[3][NS];        // LW | LX | LI | LS
-    uint8_t raw[3][NS];       // flag bytes
+    uint8_t raw[2][NS];       // flag bytes
     double dm64[2][NS];       // FP64 depth after melt
     float fe[3][NS];          // fe_1..fe_3 of column c
     float fn[2][3][NS];       // fn_1..fn_3 of columns c-1 and c
@@ -355,17 +360,17 @@ __device__ inline void store_col(const sweep_ctx &X, ring &R, const pref &a, int
     const float dm = (float)d;
     const float h = fmaxf(0.f, dm);
     const float den = __builtin_amdgcn_rcpf(fmaxf(fmaxf(h, X.hdryf), X.K.ks));
-    R.dm[SL][i] = dm;
-    R.qx[SL][i] = qx;
-    R.qy[SL][i] = qy;
+    R.dm[par][i] = dm;
+    R.qx[par][i] = qx;
+    R.qy[par][i] = qy;
     R.u[SL][i] = qx * den;
-    R.v[SL][i] = qy * den;
-    R.sq[SL][i] = sqrtf(h);
+    R.v[par][i] = qy * den;
+    R.sq[par][i] = sqrtf(h);
     R.zbh[par][i] = inner ? a.zbh : 0.f;
     R.zbl[par][i] = inner ? a.zbl : 0.f;
     R.fl[SL][i] = (uint8_t)((wet ? LW : 0) | ((a.f & FXF_WEIR) ? LX : 0) | (inner ? LI : 0) |
                             ((a.f & FXF_STALE) ? LS : 0));
-    R.raw[SL][i] = (uint8_t)a.f;
+    R.raw[par][i] = (uint8_t)a.f;
     R.dm64[par][i] = d;
 }
 
@@ -378,7 +383,7 @@ __device__ inline void flux_col(const sweep_ctx &X, ring &R, int c)
     const uint8_t fl = R.fl[S0][i];
     if (X.gate && X.flux_lane && (fl & LI) && !(fl & LX)) {
         const bool wetE = R.fl[S0][i + 1] & LW, wetN = R.fl[SN][i] & LW;
-        const int pc = c & 1, pn = pc ^ 1;
+        const int pc = c & 1, pn = pc ^ 1;     // slots of columns c and c+1
         const float dzbE = (R.zbh[pc][i + 1] - R.zbh[pc][i]) + (R.zbl[pc][i + 1] - R.zbl[pc][i]);
         const float dzbN = (R.zbh[pn][i] - R.zbh[pc][i]) + (R.zbl[pn][i] - R.zbl[pc][i]);
         if (!(fl & LW)) {
@@ -389,22 +394,21 @@ __device__ inline void flux_col(const sweep_ctx &X, ring &R, int c)
                     for (int m = 0; m < 6; m++) f[m] = X.S->f6[(uint32_t)m * (uint32_t)X.S->mm + p];
                 }
             } else {
-                solver_dry32(X.K, R.dm[S0][i], R.dm[S0][i + 1], R.dm[SN][i], dzbE, dzbN, wetE, wetN,
+                solver_dry32(X.K, R.dm[pc][i], R.dm[pc][i + 1], R.dm[pn][i], dzbE, dzbN, wetE, wetN,
                              X.dxy_over_dt, f);
             }
         } else {
             nb32 n;
-            n.dp = R.dm[S0][i]; n.de = R.dm[S0][i + 1]; n.dn = R.dm[SN][i];
-            n.dw = R.dm[S0][i - 1]; n.ds = R.dm[SS][i];
+            n.dp = R.dm[pc][i]; n.de = R.dm[pc][i + 1]; n.dn = R.dm[pn][i];
             n.dzbE = dzbE; n.dzbN = dzbN;
-            n.qp = R.qx[S0][i]; n.qe = R.qx[S0][i + 1]; n.qn = R.qx[SN][i];
-            n.rp = R.qy[S0][i]; n.re = R.qy[S0][i + 1]; n.rn = R.qy[SN][i];
-            n.uP = R.u[S0][i]; n.vP = R.v[S0][i];
+            n.qp = R.qx[pc][i]; n.qe = R.qx[pc][i + 1]; n.qn = R.qx[pn][i];
+            n.rp = R.qy[pc][i]; n.re = R.qy[pc][i + 1]; n.rn = R.qy[pn][i];
+            n.uP = R.u[S0][i]; n.vP = R.v[pc][i];
             n.uN = R.u[SN][i]; n.uS = R.u[SS][i];
             n.uSE = R.u[SS][i + 1]; n.uNE = R.u[SN][i + 1];
-            n.vE = R.v[S0][i + 1]; n.vW = R.v[S0][i - 1];
-            n.vNW = R.v[SN][i - 1]; n.vNE = R.v[SN][i + 1];
-            n.sqP = R.sq[S0][i]; n.sqE = R.sq[S0][i + 1]; n.sqN = R.sq[SN][i];
+            n.vE = R.v[pc][i + 1]; n.vW = R.v[pc][i - 1];
+            n.vNW = R.v[pn][i - 1]; n.vNE = R.v[pn][i + 1];
+            n.sqP = R.sq[pc][i]; n.sqE = R.sq[pc][i + 1]; n.sqN = R.sq[pn][i];
             n.wetE = wetE; n.wetN = wetN;
             solver_wet32(X.K, n, f);
         }
@@ -423,7 +427,7 @@ __device__ inline void update_col(const sweep_ctx &X, ring &R, int c, double &dt
     const s32p &S = *X.S;
     const int i = X.k, fs = c & 1, fsS = (c - 1) & 1;
     const uint32_t p = (uint32_t)X.row + (uint32_t)c * X.ld;
-    const uint8_t f = R.raw[S0][i];
+    const uint8_t f = R.raw[c & 1][i];
     const uint8_t fl = R.fl[S0][i];
     const bool weir = f & FXF_WEIR;
     double d = R.dm64[c & 1][i];
@@ -431,8 +435,8 @@ __device__ inline void update_col(const sweep_ctx &X, ring &R, int c, double &dt
     uint8_t nf = f;
     float fe1P = 0.f, fn1P = 0.f;
     if (!X.gate) {
-        qxn = R.qx[S0][i];
-        qyn = R.qy[S0][i];
+        qxn = R.qx[c & 1][i];
+        qyn = R.qy[c & 1][i];
     } else {
         const float kf = X.kf, dtf = X.dtf;
         const float fe1W = rflux(R, fs, 0, i - 1), fn1S = rflux(R, fsS, 3, i);
@@ -452,8 +456,8 @@ __device__ inline void update_col(const sweep_ctx &X, ring &R, int c, double &dt
             qxn = 0.f;
             qyn = 0.f;
         } else {
-            qxn = .1f * (fe1W * dtf) + .8f * (R.qx[S0][i] + dqx) + .1f * (fe1P * dtf);
-            qyn = .1f * (fn1S * dtf) + .8f * (R.qy[S0][i] + dqy) + .1f * (fn1P * dtf);
+            qxn = .1f * (fe1W * dtf) + .8f * (R.qx[c & 1][i] + dqx) + .1f * (fe1P * dtf);
+            qyn = .1f * (fn1S * dtf) + .8f * (R.qy[c & 1][i] + dqy) + .1f * (fn1P * dtf);
         }
         // keep the fluxes of a cell that may be all-dry next step (S1)
         const bool wet1 = fl & LW;
