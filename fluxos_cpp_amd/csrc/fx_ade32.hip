@@ -79,16 +79,17 @@ __device__ inline void a_load(const actx &X, int c, apref<NC> &a)
     a.f = 0;
     if (!X.row_ok || c < 0 || c >= X.mcols) return;      // column test is uniform
     const uint32_t p = (uint32_t)X.row + (uint32_t)c * X.ld;
+    const uint32_t o4 = p * 4u;
     const a32p &A = *X.A;
-    a.f = A.flags[p];
-    a.dold = X.d_old[p];
-    a.dnew = X.d_new[p];
-    a.fe = A.ff[p];
-    a.fn = A.ff[(uint32_t)A.mm + p];
-    a.mult = X.P->melt_mult ? X.P->melt_mult[p] : 1.0f;
+    a.f = at_off(A.flags, p);
+    a.dold = at_off(X.d_old, p * 8u);
+    a.dnew = at_off(X.d_new, p * 8u);
+    a.fe = at_off(A.ff, o4);
+    a.fn = at_off(A.ff + A.mm, o4);
+    a.mult = X.P->melt_mult ? at_off(X.P->melt_mult, o4) : 1.0f;
     if (p == X.inflow_p) a.f |= AF_INFLOW;
 #pragma unroll
-    for (int k = 0; k < NC; k++) a.m[k] = X.m_in[k][p];
+    for (int k = 0; k < NC; k++) a.m[k] = at_off(X.m_in[k], o4);
 }
 
 // derived values of a freshly loaded cell: geometry, and per solute the
@@ -255,7 +256,7 @@ __device__ inline void a_column(const actx &X, awin<NC> &W, int c, bool upd, boo
                 con = fmaxf(fminf(fminf(cW, crE), fminf(cs, cn)), con);
                 out = con * gP.h;
             }
-            X.m_out[k][p] = out;
+            at_off(X.m_out[k], p * 4u) = out;
         }
         W.pn_s[k] = pn;
         W.n_s[k] = n;

@@ -180,6 +180,8 @@ int check_desc(const fx_domain_desc *D)
     if (D->nchem < 0 || D->nchem > FX_MAX_CHEM) return fx_fail(FX_ERR_ARG, "nchem out of range");
     if (D->precision != FX_FP64_PARITY && D->precision != FX_FP32_MIXED)
         return fx_fail(FX_ERR_ARG, "unknown precision");
+    if (D->precision == FX_FP32_MIXED && (int64_t)(D->nrows + 2) * (D->ncols + 2) >= FX32_MAX_CELLS)
+        return fx_fail(FX_ERR_ARG, "FP32-mixed domains hold fewer than 2^29 cells (ghost rows included)");
     if (D->ade_mode < FX_ADE_OFF || D->ade_mode > FX_ADE_FAST) return fx_fail(FX_ERR_ARG, "unknown ade_mode");
     if (D->precision == FX_FP64_PARITY && D->ade_solver && D->ade_mode == FX_ADE_FAST)
         return fx_fail(FX_ERR_UNSUPPORTED, "FP64 parity domains run the exact ADE only");

@@ -326,14 +326,15 @@ __device__ inline void load_col(const sweep_ctx &X, int c, pref &a)
     a.f = 0;
     if (!X.row_ok || c < 0 || c >= X.mcols) return;      // column test is uniform
     const uint32_t p = (uint32_t)X.row + (uint32_t)c * X.ld;
+    const uint32_t o4 = p * 4u;
     const s32p &S = *X.S;
-    a.f = S.flags[p];
-    a.d = X.d_in[p];
-    a.qx = X.qx_in[p];
-    a.qy = X.qy_in[p];
-    a.zbh = S.zb[p];
-    a.zbl = S.zb_lo[p];
-    a.mult = X.P->melt_mult ? X.P->melt_mult[p] : 1.0f;
+    a.f = at_off(S.flags, p);
+    a.d = at_off(X.d_in, p * 8u);
+    a.qx = at_off(X.qx_in, o4);
+    a.qy = at_off(X.qy_in, o4);
+    a.zbh = at_off(S.zb, o4);
+    a.zbl = at_off(S.zb_lo, o4);
+    a.mult = X.P->melt_mult ? at_off(X.P->melt_mult, o4) : 1.0f;
     if (p == X.inflow_p) a.f |= PF_INFLOW;
 }
 
@@ -391,7 +392,7 @@ __device__ inline void flux_col(const sweep_ctx &X, ring &R, int c)
             if (!wetW && !wetE && !wetS && !wetN) {
                 if (fl & LS) {                     // stale fluxes of an all-dry neighbourhood
                     const uint32_t p = (uint32_t)X.row + (uint32_t)c * X.ld;
-                    for (int m = 0; m < 6; m++) f[m] = X.S->f6[(uint32_t)m * (uint32_t)X.S->mm + p];
+                    for (int m = 0; m < 6; m++) f[m] = at_off(X.S->f6 + m * X.S->mm, p * 4u);
                 }
             } else {
                 solver_dry32(X.K, R.dm[pc][i], R.dm[pc][i + 1], R.dm[pn][i], dzbE, dzbN, wetE, wetN,
@@ -427,6 +428,7 @@ __device__ inline void update_col(const sweep_ctx &X, ring &R, int c, double &dt
     const s32p &S = *X.S;
     const int i = X.k, fs = c & 1, fsS = (c - 1) & 1;
     const uint32_t p = (uint32_t)X.row + (uint32_t)c * X.ld;
+    const uint32_t o4 = p * 4u;
     const uint8_t f = R.raw[c & 1][i];
     const uint8_t fl = R.fl[S0][i];
     const bool weir = f & FXF_WEIR;
@@ -468,7 +470,7 @@ __device__ inline void update_col(const sweep_ctx &X, ring &R, int c, double &dt
                 bool nz = false;
                 for (int m = 0; m < 6; m++) nz |= rflux(R, fs, m, i) != 0.f;
                 if (nz) {
-                    for (int m = 0; m < 6; m++) S.f6[(uint32_t)m * (uint32_t)S.mm + p] = rflux(R, fs, m, i);
+                    for (int m = 0; m < 6; m++) at_off(S.f6 + m * S.mm, o4) = rflux(R, fs, m, i);
                     nf |= FXF_STALE;
                 } else {
                     nf &= ~FXF_STALE;
@@ -478,22 +480,22 @@ __device__ inline void update_col(const sweep_ctx &X, ring &R, int c, double &dt
         // wet-time tracer: runs are opened / closed on transitions only
         if (wet1 != (bool)(f & FXF_WETPREV)) {
             if (wet1) {
-                S.tw_t0[p] = (float)X.s_wet_prev;
+                at_off(S.tw_t0, o4) = (float)X.s_wet_prev;
                 nf |= FXF_WETPREV;
             } else {
-                S.tw_acc[p] += (float)(X.s_wet_prev - (double)S.tw_t0[p]);
+                at_off(S.tw_acc, o4) += (float)(X.s_wet_prev - (double)at_off(S.tw_t0, o4));
                 nf &= ~FXF_WETPREV;
             }
         }
     }
-    X.d_out[p] = d;
-    X.qx_out[p] = qxn;
-    X.qy_out[p] = qyn;
+    at_off(X.d_out, p * 8u) = d;
+    at_off(X.qx_out, o4) = qxn;
+    at_off(X.qy_out, o4) = qyn;
     if (X.ade_on) {
-        S.ff[p] = fe1P;
-        S.ff[(uint32_t)S.mm + p] = fn1P;
+        at_off(S.ff, o4) = fe1P;
+        at_off(S.ff + S.mm, o4) = fn1P;
     }
-    if (nf != f) S.flags[p] = nf;
+    if (nf != f) at_off(S.flags, p) = nf;
     cfl_cell(P, fmax(0.0, d), qxn, qyn, dtm, hmx);
 }
 

@@ -30,6 +30,22 @@ struct a32p {
 };
 
 // tiling of the FP32 kernels
+// The FP32 kernels address a cell's element as a uniform base plus a 32-bit
+// byte offset (cell index times element size), which the hardware takes as a
+// scalar base and a 32-bit vector offset.  Domains are limited to
+// FX32_MAX_CELLS so that the offset of an 8-byte element fits.
+#define FX32_MAX_CELLS ((int64_t)1 << 29)
+template <typename T>
+__device__ inline T &at_off(T *base, uint32_t off)
+{
+    return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off);
+}
+template <typename T>
+__device__ inline const T &at_off(const T *base, uint32_t off)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
+}
+
 #define FX32_TH 61            // rows owned per step-sweep strip
 #define FX32_CH 32            // default columns per step-sweep block
 #define FX32_CH_MIN 8         // smallest step-sweep block (sizes the CFL partials)

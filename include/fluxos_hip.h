@@ -98,7 +98,9 @@ typedef struct fx_step_info {
 
 /* ---- lifetime ----------------------------------------------------------
  * Replaces `GlobVar ds(NROWS+2, NCOLS+2)` (main.cpp:125-127, GlobVar.h:31-74):
- * allocates the domain's device state.                                    */
+ * allocates the domain's device state.  (NROWS+2)*(NCOLS+2) is at most 2^34
+ * cells, and below 2^29 for FX_FP32_MIXED (its kernels address a cell by a
+ * 32-bit byte offset); larger descriptors give FX_ERR_ARG.                 */
 int fx_domain_create(const fx_domain_desc *desc, fx_domain *out);
 int fx_domain_destroy(fx_domain dom);
 /* Device bytes the domain holds (for admission by memory budget). */

@@ -58,3 +58,30 @@ def test_domain_create_without_gpu_fails_loudly():
     with pytest.raises(capi.FxError) as e:
         capi.HipDomain(16, 16, 2, 1e-4)
     assert e.value.code == -8
+
+
+def _desc(n, precision):
+    d = capi.DomainDesc()
+    d.nrows = d.ncols = n
+    d.dxy, d.nchem = 1, 1
+    d.hdry, d.nodata = 1e-3, -99999.0
+    d.gacc, d.cfl, d.cvdef, d.nuem = 9.80665, 1.0, 0.07, 1.793e-6
+    d.d_coef[0] = 0.01
+    d.ade_solver = 1
+    d.precision = precision
+    d.ade_mode = capi.ADE_FAST if precision == capi.FP32_MIXED else capi.ADE_EXACT
+    return d
+
+
+def test_fp32_domain_size_limit(fxlib):
+    """The FP32-mixed kernels address cells by 32-bit byte offsets: a domain of
+    2^29 cells or more (ghost rows included) is refused before any allocation;
+    the FP64 path is not limited by it.  (NROWS + 2)^2 crosses 2^29 between
+    NROWS = 23168 and 23169."""
+    import ctypes as C
+    below, above = _desc(23168, capi.FP32_MIXED), _desc(23169, capi.FP32_MIXED)
+    assert (23168 + 2) ** 2 < 1 << 29 <= (23169 + 2) ** 2
+    assert fxlib.fx_domain_bytes(C.byref(below)) > 0
+    assert fxlib.fx_domain_bytes(C.byref(above)) == -1
+    assert b"2^29" in fxlib.fx_last_error()
+    assert fxlib.fx_domain_bytes(C.byref(_desc(23169, capi.FP64_PARITY))) > 0
