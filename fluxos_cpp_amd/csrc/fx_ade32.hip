@@ -315,6 +315,18 @@ __device__ inline void ade_tile(const fx_params &P, const a32p &A, fx_ctl *ctl, 
     a_shift<NC>(W);
     a_load<NC>(X, c0 + 1, a); a_derive<NC>(X, a, W.g[3], W.c1[3], W.cr[3]);
     a_column<NC>(X, W, c0 - 1, false, own);
+    if constexpr (NC >= 3) {
+        // three or four solutes: one column of prefetch (a: c+2) keeps the
+        // kernel at five waves per SIMD, which large grids of many rounds need
+        a_load<NC>(X, c0 + 2, a);
+        for (int c = c0; c <= c_last; c++) {
+            a_shift<NC>(W);
+            a_derive<NC>(X, a, W.g[3], W.c1[3], W.cr[3]);
+            if (c + 3 <= c_last + 2) a_load<NC>(X, c + 3, a);
+            a_column<NC>(X, W, c, true, own);
+        }
+        return;
+    }
     // steady state: prefetch two columns ahead (a: c+2, b: c+3)
     a_load<NC>(X, c0 + 2, a);
     a_load<NC>(X, c0 + 3, b);
