@@ -188,6 +188,9 @@ void orc_solver_dry(orc_domain *d, int irow, int icol)
     if (ldw == 1 && ldp == 1 && lde == 1 && lds == 1 && ldn == 1) {
         d->dh[p] = 0.0; d->dqx[p] = 0.0; d->dqy[p] = 0.0;
         d->qxf[p] = 0.0; d->qyf[p] = 0.0; d->qx[p] = 0.0; d->qy[p] = 0.0;
+#ifdef ORC_PLANT_STALE
+        d->fe1[p] = d->fe2[p] = d->fe3[p] = d->fn1[p] = d->fn2[p] = d->fn3[p] = 0.0;
+#endif
         return;
     }
     const double zbp = d->zb[p], zbe = d->zb[pe], zbn = d->zb[pn];
@@ -389,9 +392,11 @@ void orc_solver_wet(orc_domain *d, int irow, int icol)
         if (hme > d->hdry) {
             double dzea = fabs(dze);
             if (dzea > 0.5 * hme) {
+#ifndef ORC_PLANT_DZ_LIMIT
                 double dhea = fabs(he - hp);
                 dzea = fmin(dzea, dhea);
                 dze = copysign(dzea, dze);
+#endif
             }
             cc = .25 / cme;
         } else {
@@ -400,12 +405,20 @@ void orc_solver_wet(orc_domain *d, int irow, int icol)
         c1 = ume; c2 = ume + cme; c3 = ume - cme;
         c1a = fabs(c1); c2a = fabs(c2); c3a = fabs(c3);
         a11 = c2 * c3a - c2a * c3;
+#ifdef ORC_PLANT_ROE_ABS
+        a12 = c2 - c3;
+#else
         a12 = c2a - c3a;
+#endif
         a21 = c2 * c3 * (c3a - c2a);
         a22 = c2a * c2 - c3a * c3;
         a31 = vme * (c2 * c3a - 2. * cme * c1a - c2a * c3);
         a32 = vme * (c2a - c3a);
+#ifdef ORC_PLANT_ROE_SHEAR
+        a33 = 2. * cme * c1;
+#else
         a33 = 2. * cme * c1a;
+#endif
 #ifdef ORC_PLANT_ROE_SIGN
         fe1r = (a11 * dze + a12 * dqe) * cc;
 #else
@@ -729,8 +742,10 @@ void orc_adesolver(orc_domain *d, int it, int ichem, double *scratch)
         /* new concentration (:280-288) */
         const double dc = (pfw - pfe + qfs - qfn) * dtfl / area;
         double con = c[p] + dc / hp;
+#ifndef ORC_PLANT_ADE_CLAMP
         con = fmin(cmaxr[p], con);
         con = fmax(cminr[p], con);
+#endif
         c[p] = con;
         qfcds[ix] = qfn;
     }

@@ -51,13 +51,20 @@
  *   orc_wintra         geometric decay m (1 - r dt/3600)^n of the soil mass
  *                      in a still wet basin, soil + solute mass conserved,
  *                      dry and NODATA cells untouched
+ *   orc_solver_wet     also with moving water: pure upwind mass flux in
+ *                      supercritical flow, a shear wave damped by 1/2 |u|
+ *                      times its jump, the |dz| cap at a bed step
+ *                      (east faces above half the depth, north faces always)
+ *   orc_solver_dry     stale fluxes of an all-dry neighbourhood re-applied
+ *                      at every step
+ *   orc_adesolver      new values within the neighbours' bounds under
+ *                      strong transport
  * Parity unpinned (no closed form and no reference vectors): the Roe
- * dissipation with a nonzero face velocity and its |dz| limiter branches
- * (solver_wetdomain.cpp:216-229, 249-259); the stale-flux persistence of
- * all-dry neighbourhoods over several steps (solver_drydomain.cpp:60-76);
- * the ADE's outflow limiter branches and min/max clamp under strong
- * transport (ADEsolver_calc.cpp:243-285).  These follow the reference text
- * line by line and are covered only by the GPU-vs-oracle parity tests.
+ * dissipation of a subcritical moving flow across a free-surface step, where
+ * the matrix mixes the three waves (solver_wetdomain.cpp:231-247, 260-276);
+ * the ADE's outflow-limiter branches one by one (ADEsolver_calc.cpp:244-278).
+ * These follow the reference text line by line and are covered only by the
+ * GPU-vs-oracle parity tests.
  */
 #ifndef FLUXOS_ORACLE_H
 #define FLUXOS_ORACLE_H

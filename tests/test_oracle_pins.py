@@ -14,6 +14,12 @@ physics and the reference's documented behaviour fix independently:
   front, for a lone wet cell and for a dry cell next to a wet one;
 * Roe's upwind mass flux -1/2 c dz across a still free-surface step, and
   the eddy-viscosity shear stress nu H dv/dx in a linear shear flow;
+* Roe's scheme with moving water: pure upwinding of the mass flux in
+  supercritical flow, the shear wave damped by 1/2 |u| times its jump, and
+  the cap of the dissipated step at the depth difference over a bed step;
+* stale fluxes of an all-dry neighbourhood applied again at every step;
+* the ADE's new values within their neighbours' bounds under strong
+  transport;
 * conservation: water in a walled basin, solute under pure transport;
 * diffusion: one ADE sweep with still water is the explicit 5-point scheme
   c + 2 D dt / dx^2 * Laplacian(c) (sigc = 0.5), exact on a quadratic;
@@ -51,6 +57,130 @@ def flat_domain(n=12, dxy=2, h=0.0, hdry=1e-4, ade=True, nchem=1, lib=None, bed=
     d.h[...] = np.maximum(d.z - d.zb, 0.0)
     d.ldry[1:-1, 1:-1] = np.where(d.h[1:-1, 1:-1] > hdry, 0.0, 1.0)
     return d
+
+
+# ------------------------------------------------- Roe with moving water ----
+def check_roe_supercritical(lib=None):
+    """Uniform depth over a flat bed, flowing faster than the wave speed
+    (|u| > sqrt(g H)), with a jump in discharge across one face: every
+    characteristic speed has the sign of u, so Roe's scheme is pure upwind and
+    the face carries the upwind cell's discharge as its mass flux
+    (solver_wetdomain.cpp:213-285), along both axes and in both directions."""
+    H, n = 0.2, 12
+    c = math.sqrt(G * H)
+    for axis in (0, 1):
+        for sgn in (1.0, -1.0):
+            u0, u1 = sgn * 2.6 * c, sgn * 2.9 * c
+            d = flat_domain(n=n, h=H, lib=lib, bed=0.0)
+            q = d.qx if axis == 0 else d.qy
+            q[...] = H * u0
+            if axis == 0:
+                q[6:, :] = H * u1
+            else:
+                q[:, 6:] = H * u1
+            d.dtfl = 1e-3
+            d.hydrodynamics()
+            got = d.fe1[5, 3:-3] if axis == 0 else d.fn1[3:-3, 5]
+            want = H * (u0 if sgn > 0 else u1)
+            assert np.allclose(got, want, rtol=1e-13, atol=0), (axis, sgn, got[:3], want)
+
+
+def check_roe_shear_wave(lib=None):
+    """Uniform depth and normal velocity u0 over a flat bed, with a jump in the
+    transverse velocity across one face (a shear wave, speed u0): Roe's
+    dissipation of the transverse-momentum flux is 1/2 |u0| times the jump, so
+    the face carries H u0 v of the upwind side, less the eddy-viscosity stress
+    nu H dv/dx (solver_wetdomain.cpp:177-285), along both axes and in both
+    directions."""
+    H, n, dxy, v0, v1 = 0.2, 12, 2.0, 0.1, -0.05
+    nu = float(np.float32(1.793e-6))
+    for axis in (0, 1):
+        for u0 in (0.3, -0.3):
+            d = flat_domain(n=n, h=H, lib=lib, bed=0.0)
+            qn, qt = (d.qx, d.qy) if axis == 0 else (d.qy, d.qx)
+            qn[...] = H * u0
+            qt[...] = H * v0
+            if axis == 0:
+                qt[6:, :] = H * v1
+            else:
+                qt[:, 6:] = H * v1
+            d.dtfl = 1e-3
+            d.hydrodynamics()
+            want = H * u0 * (v0 if u0 > 0 else v1) - nu * H * (v1 - v0) / dxy
+            got = d.fe3[5, 3:-3] if axis == 0 else d.fn2[3:-3, 5]
+            assert np.allclose(got, want, rtol=1e-12, atol=0), (axis, u0, got[:3], want)
+
+
+def check_roe_dz_limiter(lib=None):
+    """Still water of equal depth H either side of a bed step: the free surface
+    steps by the bed step dz.  Roe's still-step mass flux -1/2 c dz applies on
+    an east face while |dz| <= H/2; beyond that the reference caps the step it
+    dissipates at the depth difference, here zero, and on north faces it always
+    does (solver_wetdomain.cpp:216-225, 249-255).  A capped face carries no
+    mass and only the hydrostatic thrust 1/2 g H^2."""
+    H, n = 0.3, 10
+    for axis, dz, capped in ((0, 0.1, False), (0, 0.2, True), (1, 0.1, True)):
+        d = flat_domain(n=n, h=H, lib=lib, bed=0.0)
+        if axis == 0:
+            d.zb[6:-1, :] = dz
+        else:
+            d.zb[:, 6:-1] = dz
+        d.z[...] = d.zb + H
+        d.h[...] = H
+        d.dtfl = 1e-3
+        d.hydrodynamics()
+        m = d.fe1[5, 3:-3] if axis == 0 else d.fn1[3:-3, 5]
+        p = d.fe2[5, 3:-3] if axis == 0 else d.fn3[3:-3, 5]
+        want = 0.0 if capped else -0.5 * math.sqrt(G * H) * dz
+        assert np.allclose(m, want, rtol=1e-12, atol=1e-16), (axis, dz, m[:3], want)
+        if capped:
+            assert np.allclose(p, 0.5 * G * H * H, rtol=1e-12, atol=0), (axis, dz, p[:3])
+
+
+def check_stale_fluxes_persist(lib=None):
+    """A dry cell whose four neighbours are dry keeps the face fluxes it last
+    had (solver_drydomain.cpp:60-76), and the update applies them again at
+    every step: the water surface of the cell and of its east neighbour drift
+    linearly, by -/+ dt fe / dx per step, while the neighbourhood stays dry."""
+    n, dxy, F, dt, steps = 10, 2, 1e-6, 0.5, 6
+    d = flat_domain(n=n, h=0.0, lib=lib, bed=0.0)
+    d.fe1[5, 5] = F
+    d.dtfl = dt
+    z0 = d.z.copy()
+    for k in range(1, steps + 1):
+        d.hydrodynamics()
+        step = k * dt * F / dxy
+        assert d.z[5, 5] == pytest.approx(z0[5, 5] - step, rel=1e-12), k
+        assert d.z[6, 5] == pytest.approx(z0[6, 5] + step, rel=1e-12), k
+    moved = np.argwhere(d.z != z0)
+    assert sorted(map(tuple, moved)) == [(5, 5), (6, 5)]
+
+
+def check_ade_clamp(lib=None):
+    """Strong transport of random sign (face Courant numbers up to 0.75)
+    through uniform water carrying a random field: whatever the fluxes, each
+    new concentration lies within the smallest and largest of its four
+    neighbours' values (ADEsolver_calc.cpp:47-62, 284-285).  No cell changes
+    depth (the cells count as dry before the step, so pass 1 leaves them as
+    they are), so those values are the step's inputs."""
+    n = 14
+    rng = np.random.default_rng(5)
+    d = _ade_box(lib, lambda i, j: rng.random(i.shape),
+                 fe=lambda i, j: rng.uniform(-0.15, 0.15, i.shape),
+                 fn=lambda i, j: rng.uniform(-0.15, 0.15, i.shape), n=n)
+    d.ldry_prev[...] = 1.0
+    c0 = d.conc[0].copy()
+    d.adesolver(2, 0)
+    c1 = d.conc[0]
+    changed = 0
+    for iy in range(1, n + 1):
+        for ix in range(1, n + 1):
+            if d.ldry[ix, iy] != 0.0 or (ix, iy) == (n, n):
+                continue                       # dry cells zero; the last cell is not visited
+            nb = (c0[ix - 1, iy], c0[ix + 1, iy], c0[ix, iy - 1], c0[ix, iy + 1])
+            assert min(nb) <= c1[ix, iy] <= max(nb), (ix, iy, c1[ix, iy], nb)
+            changed += c1[ix, iy] != c0[ix, iy]
+    assert changed > (n - 2) ** 2 // 2
 
 
 # ---------------------------------------------------------------- CFL ----
@@ -625,3 +755,23 @@ def test_restart_row_outside_grid_is_refused():
     rows[1, 0] = 9
     with pytest.raises(ValueError):
         d.initiation_restart(rows)
+
+
+def test_roe_upwind_mass_flux_in_supercritical_flow():
+    check_roe_supercritical()
+
+
+def test_roe_shear_wave_dissipation():
+    check_roe_shear_wave()
+
+
+def test_roe_step_limiter_at_a_bed_step():
+    check_roe_dz_limiter()
+
+
+def test_stale_fluxes_persist_in_a_dry_neighbourhood():
+    check_stale_fluxes_persist()
+
+
+def test_ade_new_values_within_neighbour_bounds():
+    check_ade_clamp()

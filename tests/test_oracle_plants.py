@@ -30,6 +30,11 @@ PLANTS = [
     ("CONV", pins.check_uniform_flow),            # x-momentum carried with v instead of u
     ("WINTRA_AREA", pins.check_wintra),           # released mass divided by h only, not h dxy^2
     ("RESTART_QX", pins.check_restart),           # q*dxy read back divided by dxy
+    ("ROE_ABS", pins.check_roe_supercritical),    # |c2| - |c3| in the mass dissipation taken without the absolute values
+    ("ROE_SHEAR", pins.check_roe_shear_wave),     # the shear wave's speed |u| taken as u
+    ("DZ_LIMIT", pins.check_roe_dz_limiter),      # east-face step not capped at the depth difference
+    ("STALE", pins.check_stale_fluxes_persist),   # fluxes of an all-dry neighbourhood zeroed
+    ("ADE_CLAMP", pins.check_ade_clamp),          # new concentration not clamped to the neighbour bounds
 ]
 
 
