@@ -410,7 +410,11 @@ void orc_solver_wet(orc_domain *d, int irow, int icol)
 #else
         a12 = c2a - c3a;
 #endif
+#ifdef ORC_PLANT_ROE_A21
+        a21 = c2 * c3 * (c2a - c3a);
+#else
         a21 = c2 * c3 * (c3a - c2a);
+#endif
         a22 = c2a * c2 - c3a * c3;
         a31 = vme * (c2 * c3a - 2. * cme * c1a - c2a * c3);
         a32 = vme * (c2a - c3a);
@@ -716,7 +720,11 @@ void orc_adesolver(orc_domain *d, int it, int ichem, double *scratch)
         const double cvolrat = cvolpot / dtfl + pfw + qfs;
         if (cvolrat > 0.0) {
             if (pfe > 0. && qfn > 0.) {
+#ifdef ORC_PLANT_ADE_LIMIT
+                if (0) {
+#else
                 if (pfe + qfn > cvolrat) {
+#endif
                     double cf = qfn / (pfe + qfn);
                     pfe = (1. - cf) * cvolrat;
                     qfn = cf * cvolrat;

@@ -51,20 +51,22 @@
  *   orc_wintra         geometric decay m (1 - r dt/3600)^n of the soil mass
  *                      in a still wet basin, soil + solute mass conserved,
  *                      dry and NODATA cells untouched
- *   orc_solver_wet     also with moving water: pure upwind mass flux in
- *                      supercritical flow, a shear wave damped by 1/2 |u|
- *                      times its jump, the |dz| cap at a bed step
+ *   orc_solver_wet     also with moving water: the Roe dissipation is
+ *                      -1/2 |A| dU (A the flux Jacobian at the face's mean
+ *                      state, |A| by eigendecomposition) on random
+ *                      subcritical and supercritical states; pure upwind
+ *                      mass flux in supercritical flow; a shear wave damped
+ *                      by 1/2 |u| times its jump; the |dz| cap at a bed step
  *                      (east faces above half the depth, north faces always)
  *   orc_solver_dry     stale fluxes of an all-dry neighbourhood re-applied
  *                      at every step
  *   orc_adesolver      new values within the neighbours' bounds under
- *                      strong transport
- * Parity unpinned (no closed form and no reference vectors): the Roe
- * dissipation of a subcritical moving flow across a free-surface step, where
- * the matrix mixes the three waves (solver_wetdomain.cpp:231-247, 260-276);
- * the ADE's outflow-limiter branches one by one (ADEsolver_calc.cpp:244-278).
- * These follow the reference text line by line and are covered only by the
- * GPU-vs-oracle parity tests.
+ *                      strong transport; the outflow limiter splitting the
+ *                      held solute a : b between two outflowing faces
+ * Parity unpinned (no closed form and no reference vectors): the ADE outflow
+ * limiter's branches with a single outflowing face or an inflow-dominated
+ * cell (ADEsolver_calc.cpp:259-278).  They follow the reference text line by
+ * line and are covered only by the GPU-vs-oracle parity tests.
  */
 #ifndef FLUXOS_ORACLE_H
 #define FLUXOS_ORACLE_H

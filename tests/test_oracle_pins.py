@@ -14,12 +14,15 @@ physics and the reference's documented behaviour fix independently:
   front, for a lone wet cell and for a dry cell next to a wet one;
 * Roe's upwind mass flux -1/2 c dz across a still free-surface step, and
   the eddy-viscosity shear stress nu H dv/dx in a linear shear flow;
-* Roe's scheme with moving water: pure upwinding of the mass flux in
-  supercritical flow, the shear wave damped by 1/2 |u| times its jump, and
-  the cap of the dissipated step at the depth difference over a bed step;
+* Roe's scheme with moving water: the dissipation is -1/2 |A| dU with the
+  flux Jacobian at the face's mean state (|A| by numpy's eigendecomposition),
+  pure upwinding of the mass flux in supercritical flow, the shear wave
+  damped by 1/2 |u| times its jump, and the cap of the dissipated step at
+  the depth difference over a bed step;
 * stale fluxes of an all-dry neighbourhood applied again at every step;
 * the ADE's new values within their neighbours' bounds under strong
-  transport;
+  transport, and its outflow limiter splitting a cell's solute a : b between
+  two outflowing faces;
 * conservation: water in a walled basin, solute under pure transport;
 * diffusion: one ADE sweep with still water is the explicit 5-point scheme
   c + 2 D dt / dx^2 * Laplacian(c) (sigc = 0.5), exact on a quadratic;
@@ -111,6 +114,50 @@ def check_roe_shear_wave(lib=None):
             assert np.allclose(got, want, rtol=1e-12, atol=0), (axis, u0, got[:3], want)
 
 
+def _abs_matrix(a):
+    """|A| = R |Lambda| R^-1 by eigendecomposition (A has real eigenvalues)."""
+    w, r = np.linalg.eig(a)
+    return (r @ np.diag(np.abs(w)) @ np.linalg.inv(r)).real
+
+
+def check_roe_matrix(lib=None):
+    """Random subcritical and supercritical states that vary along one axis
+    over a flat bed, with a uniform transverse velocity (so no eddy stress):
+    at every face the Roe dissipation, the face flux less its convective and
+    pressure parts, is -1/2 |A| dU, with A the Jacobian of the shallow-water
+    flux at the face's mean state (h, q, r averaged) and dU the jump of
+    (h, q, r) across the face (solver_wetdomain.cpp:213-285)."""
+    n, H = 12, 0.3
+    rng = np.random.default_rng(7)
+    c = math.sqrt(G * H)
+    for axis in (0, 1):
+        d = flat_domain(n=n, h=H, lib=lib, bed=0.0)
+        hv = H * (1.0 + 0.08 * rng.uniform(-1, 1, n + 2))        # |dh| < h/2: no step cap
+        un = c * rng.uniform(-1.6, 1.6, n + 2)                    # normal velocity
+        vt = 0.07                                                 # transverse velocity
+        hh = hv[:, None] if axis == 0 else hv[None, :]
+        uu = un[:, None] if axis == 0 else un[None, :]
+        d.h[...] = hh + 0.0 * d.h
+        d.z[...] = d.zb + d.h
+        (d.qx if axis == 0 else d.qy)[...] = d.h * uu
+        (d.qy if axis == 0 else d.qx)[...] = d.h * vt
+        d.dtfl = 1e-4
+        h0, qn0, qt0 = d.h.copy(), (d.qx if axis == 0 else d.qy).copy(), (d.qy if axis == 0 else d.qx).copy()
+        d.hydrodynamics()
+        for k in range(3, n - 2):
+            p, e = ((k, 5), (k + 1, 5)) if axis == 0 else ((5, k), (5, k + 1))
+            hm = 0.5 * (h0[p] + h0[e])
+            qm, rm = 0.5 * (qn0[p] + qn0[e]), 0.5 * (qt0[p] + qt0[e])
+            um, vm, cm = qm / hm, rm / hm, math.sqrt(G * hm)
+            jac = np.array([[0.0, 1.0, 0.0], [cm * cm - um * um, 2 * um, 0.0], [-um * vm, vm, um]])
+            want = -0.5 * _abs_matrix(jac) @ np.array([h0[e] - h0[p], qn0[e] - qn0[p], qt0[e] - qt0[p]])
+            if axis == 0:
+                got = (d.fe1[p] - qm, d.fe2[p] - qm * um - 0.5 * G * hm * hm, d.fe3[p] - qm * vm)
+            else:
+                got = (d.fn1[p] - qm, d.fn3[p] - qm * um - 0.5 * G * hm * hm, d.fn2[p] - qm * vm)
+            assert np.allclose(got, want, rtol=1e-9, atol=1e-14), (axis, k, got, want)
+
+
 def check_roe_dz_limiter(lib=None):
     """Still water of equal depth H either side of a bed step: the free surface
     steps by the bed step dz.  Roe's still-step mass flux -1/2 c dz applies on
@@ -154,6 +201,25 @@ def check_stale_fluxes_persist(lib=None):
         assert d.z[6, 5] == pytest.approx(z0[6, 5] + step, rel=1e-12), k
     moved = np.argwhere(d.z != z0)
     assert sorted(map(tuple, moved)) == [(5, 5), (6, 5)]
+
+
+def check_ade_outflow_split(lib=None):
+    """One cell holding solute in still, solute-free water, with east and north
+    face discharges a, b that would carry out more than it holds (no
+    diffusion): the outflow limiter sends out exactly what it holds, split
+    a : b between the two faces (ADEsolver_calc.cpp:243-259), so the east and
+    north neighbours end at a/(a+b) and b/(a+b) of its concentration and the
+    cell itself at zero."""
+    n, cP, a, b = 14, 0.8, 0.3, 0.2
+    d = _ade_box(lib, lambda i, j: 0.0 * i, D=0.0, n=n)
+    d.conc[0][6, 6] = cP
+    d.fe1[6, 6], d.fn1[6, 6] = a, b
+    d.ldry_prev[...] = 1.0                     # no rescale: depths do not change
+    d.adesolver(2, 0)
+    c = d.conc[0]
+    assert c[7, 6] == pytest.approx(a / (a + b) * cP, rel=1e-12)
+    assert c[6, 7] == pytest.approx(b / (a + b) * cP, rel=1e-12)
+    assert abs(c[6, 6]) < 1e-15
 
 
 def check_ade_clamp(lib=None):
@@ -775,3 +841,11 @@ def test_stale_fluxes_persist_in_a_dry_neighbourhood():
 
 def test_ade_new_values_within_neighbour_bounds():
     check_ade_clamp()
+
+
+def test_roe_dissipation_is_half_the_jacobian_modulus():
+    check_roe_matrix()
+
+
+def test_ade_outflow_limiter_splits_the_held_solute():
+    check_ade_outflow_split()

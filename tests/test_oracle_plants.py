@@ -35,6 +35,8 @@ PLANTS = [
     ("DZ_LIMIT", pins.check_roe_dz_limiter),      # east-face step not capped at the depth difference
     ("STALE", pins.check_stale_fluxes_persist),   # fluxes of an all-dry neighbourhood zeroed
     ("ADE_CLAMP", pins.check_ade_clamp),          # new concentration not clamped to the neighbour bounds
+    ("ROE_A21", pins.check_roe_matrix),           # (c3a - c2a) written as (c2a - c3a) in the momentum row
+    ("ADE_LIMIT", pins.check_ade_outflow_split),  # outflow of both faces not limited to the held solute
 ]
 
 
