@@ -730,7 +730,9 @@ void orc_adesolver(orc_domain *d, int it, int ichem, double *scratch)
                     qfn = cf * cvolrat;
                 }
             } else if (pfe > 0.) {
+#ifndef ORC_PLANT_ADE_LIMIT1
                 pfe = fmin(pfe, (cvolrat - qfn));
+#endif
             } else if (qfn > 0.) {
                 qfn = fmin(qfn, (cvolrat - pfe));
             }

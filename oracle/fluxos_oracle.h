@@ -61,12 +61,13 @@
  *   orc_solver_dry     stale fluxes of an all-dry neighbourhood re-applied
  *                      at every step
  *   orc_adesolver      new values within the neighbours' bounds under
- *                      strong transport; the outflow limiter splitting the
- *                      held solute a : b between two outflowing faces
+ *                      strong transport; the outflow limiter sending out
+ *                      exactly the held solute through one face (east or
+ *                      north), or split a : b between two
  * Parity unpinned (no closed form and no reference vectors): the ADE outflow
- * limiter's branches with a single outflowing face or an inflow-dominated
- * cell (ADEsolver_calc.cpp:259-278).  They follow the reference text line by
- * line and are covered only by the GPU-vs-oracle parity tests.
+ * limiter's branches for a cell whose inflow terms are negative
+ * (ADEsolver_calc.cpp:259-278).  They follow the reference text line by line
+ * and are covered only by the GPU-vs-oracle parity tests.
  */
 #ifndef FLUXOS_ORACLE_H
 #define FLUXOS_ORACLE_H

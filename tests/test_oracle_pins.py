@@ -21,8 +21,8 @@ physics and the reference's documented behaviour fix independently:
   the depth difference over a bed step;
 * stale fluxes of an all-dry neighbourhood applied again at every step;
 * the ADE's new values within their neighbours' bounds under strong
-  transport, and its outflow limiter splitting a cell's solute a : b between
-  two outflowing faces;
+  transport, and its outflow limiter sending out exactly the held solute
+  through one face, or split a : b between two;
 * conservation: water in a walled basin, solute under pure transport;
 * diffusion: one ADE sweep with still water is the explicit 5-point scheme
   c + 2 D dt / dx^2 * Laplacian(c) (sigc = 0.5), exact on a quadratic;
@@ -207,7 +207,7 @@ def check_ade_outflow_split(lib=None):
     """One cell holding solute in still, solute-free water, with east and north
     face discharges a, b that would carry out more than it holds (no
     diffusion): the outflow limiter sends out exactly what it holds, split
-    a : b between the two faces (ADEsolver_calc.cpp:243-259), so the east and
+    a : b between the two faces (ADEsolver_calc.cpp:244-253), so the east and
     north neighbours end at a/(a+b) and b/(a+b) of its concentration and the
     cell itself at zero."""
     n, cP, a, b = 14, 0.8, 0.3, 0.2
@@ -220,6 +220,26 @@ def check_ade_outflow_split(lib=None):
     assert c[7, 6] == pytest.approx(a / (a + b) * cP, rel=1e-12)
     assert c[6, 7] == pytest.approx(b / (a + b) * cP, rel=1e-12)
     assert abs(c[6, 6]) < 1e-15
+
+
+def check_ade_single_outflow(lib=None):
+    """One cell holding solute in still water (no diffusion) whose east (or
+    north) face discharge would carry out more than it holds: the limiter lets
+    the face carry exactly the held solute (ADEsolver_calc.cpp:254-257), so the
+    receiving neighbour gains the whole of it, c_Q + c_P at equal depths, and
+    the cell ends at zero.  A cell beyond the receiver holds more solute, so
+    the receiver's own bounds do not cap its value."""
+    n, cP, cQ, a = 14, 0.5, 0.1, 0.3
+    for axis in (0, 1):
+        d = _ade_box(lib, lambda i, j: 0.0 * i, D=0.0, n=n)
+        q, qq = ((7, 6), (8, 6)) if axis == 0 else ((6, 7), (6, 8))
+        d.conc[0][6, 6], d.conc[0][q], d.conc[0][qq] = cP, cQ, 1.0
+        (d.fe1 if axis == 0 else d.fn1)[6, 6] = a
+        d.ldry_prev[...] = 1.0                 # no rescale: depths do not change
+        d.adesolver(2, 0)
+        c = d.conc[0]
+        assert c[q] == pytest.approx(cQ + cP, rel=1e-12), axis
+        assert abs(c[6, 6]) < 1e-15, axis
 
 
 def check_ade_clamp(lib=None):
@@ -849,3 +869,7 @@ def test_roe_dissipation_is_half_the_jacobian_modulus():
 
 def test_ade_outflow_limiter_splits_the_held_solute():
     check_ade_outflow_split()
+
+
+def test_ade_outflow_limiter_single_face():
+    check_ade_single_outflow()

@@ -37,6 +37,7 @@ PLANTS = [
     ("ADE_CLAMP", pins.check_ade_clamp),          # new concentration not clamped to the neighbour bounds
     ("ROE_A21", pins.check_roe_matrix),           # (c3a - c2a) written as (c2a - c3a) in the momentum row
     ("ADE_LIMIT", pins.check_ade_outflow_split),  # outflow of both faces not limited to the held solute
+    ("ADE_LIMIT1", pins.check_ade_single_outflow),  # outflow of a single face not limited
 ]
 
 
