@@ -746,7 +746,9 @@ void orc_adesolver(orc_domain *d, int it, int ichem, double *scratch)
                 if (cbilan > 0.) qfn = fmin(qfn, cbilan);
                 else qfn = 0.;
             } else if (pfe >= 0. && qfn >= 0.) {
+#ifndef ORC_PLANT_ADE_BILAN
                 pfe = 0.0; qfn = 0.0;
+#endif
             }
         }
         /* new concentration (:280-288) */

@@ -63,10 +63,12 @@
  *   orc_adesolver      new values within the neighbours' bounds under
  *                      strong transport; the outflow limiter sending out
  *                      exactly the held solute through one face (east or
- *                      north), or split a : b between two
+ *                      north), or split a : b between two, and nothing from
+ *                      a cell another face has emptied
  * Parity unpinned (no closed form and no reference vectors): the ADE outflow
- * limiter's branches for a cell whose inflow terms are negative
- * (ADEsolver_calc.cpp:259-278).  They follow the reference text line by line
+ * limiter's branches for a cell emptied through its west or south face that
+ * also takes solute in through its east or north face
+ * (ADEsolver_calc.cpp:260-273).  They follow the reference text line by line
  * and are covered only by the GPU-vs-oracle parity tests.
  */
 #ifndef FLUXOS_ORACLE_H

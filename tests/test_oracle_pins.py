@@ -22,7 +22,8 @@ physics and the reference's documented behaviour fix independently:
 * stale fluxes of an all-dry neighbourhood applied again at every step;
 * the ADE's new values within their neighbours' bounds under strong
   transport, and its outflow limiter sending out exactly the held solute
-  through one face, or split a : b between two;
+  through one face, or split a : b between two, and nothing from a cell
+  another face has emptied;
 * conservation: water in a walled basin, solute under pure transport;
 * diffusion: one ADE sweep with still water is the explicit 5-point scheme
   c + 2 D dt / dx^2 * Laplacian(c) (sigc = 0.5), exact on a quadratic;
@@ -240,6 +241,26 @@ def check_ade_single_outflow(lib=None):
         c = d.conc[0]
         assert c[q] == pytest.approx(cQ + cP, rel=1e-12), axis
         assert abs(c[6, 6]) < 1e-15, axis
+
+
+def check_ade_emptied_cell(lib=None):
+    """Still water (no diffusion); a cell P holding solute, whose west face
+    discharge flows west strongly enough to take all of it (the west cell's
+    face flux is capped at what P holds), while P's own east face discharge
+    points east: P has nothing left to send, so the reference cancels its
+    east and north outflows (ADEsolver_calc.cpp:259, 274-276).  The west cell
+    ends with P's concentration, P and its east neighbour at zero."""
+    n, cP = 14, 0.5
+    d = _ade_box(lib, lambda i, j: 0.0 * i, D=0.0, n=n)
+    d.conc[0][6, 6] = cP
+    d.fe1[5, 6] = -0.5                         # west face of P: flow from P to the west
+    d.fe1[6, 6] = 0.3                          # east face of P: flow to the east
+    d.ldry_prev[...] = 1.0                     # no rescale: depths do not change
+    d.adesolver(2, 0)
+    c = d.conc[0]
+    assert c[5, 6] == pytest.approx(cP, rel=1e-12)
+    assert abs(c[6, 6]) < 1e-15
+    assert c[7, 6] == 0.0
 
 
 def check_ade_clamp(lib=None):
@@ -873,3 +894,7 @@ def test_ade_outflow_limiter_splits_the_held_solute():
 
 def test_ade_outflow_limiter_single_face():
     check_ade_single_outflow()
+
+
+def test_ade_emptied_cell_sends_nothing_on():
+    check_ade_emptied_cell()

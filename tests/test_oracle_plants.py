@@ -38,6 +38,7 @@ PLANTS = [
     ("ROE_A21", pins.check_roe_matrix),           # (c3a - c2a) written as (c2a - c3a) in the momentum row
     ("ADE_LIMIT", pins.check_ade_outflow_split),  # outflow of both faces not limited to the held solute
     ("ADE_LIMIT1", pins.check_ade_single_outflow),  # outflow of a single face not limited
+    ("ADE_BILAN", pins.check_ade_emptied_cell),   # outflows of an emptied cell not cancelled
 ]
 
 
