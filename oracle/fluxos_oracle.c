@@ -739,7 +739,11 @@ void orc_adesolver(orc_domain *d, int it, int ichem, double *scratch)
         } else {
             if (pfe >= 0. && qfn < 0.) {
                 double cbilan = cvolrat - qfn;
+#ifdef ORC_PLANT_ADE_PASS
+                if (cbilan > 0.) pfe = pfe;
+#else
                 if (cbilan > 0.) pfe = fmin(pfe, cbilan);
+#endif
                 else pfe = 0.;
             } else if (pfe < 0. && qfn >= 0.) {
                 double cbilan = cvolrat - pfe;

@@ -63,13 +63,13 @@
  *   orc_adesolver      new values within the neighbours' bounds under
  *                      strong transport; the outflow limiter sending out
  *                      exactly the held solute through one face (east or
- *                      north), or split a : b between two, and nothing from
- *                      a cell another face has emptied
- * Parity unpinned (no closed form and no reference vectors): the ADE outflow
- * limiter's branches for a cell emptied through its west or south face that
- * also takes solute in through its east or north face
- * (ADEsolver_calc.cpp:260-273).  They follow the reference text line by line
- * and are covered only by the GPU-vs-oracle parity tests.
+ *                      north), or split a : b between two; nothing from a
+ *                      cell another face has emptied, and from such a cell
+ *                      exactly what it takes in through a third face
+ * Parity unpinned (no closed form and no reference vectors): the two
+ * sub-branches where an emptied cell's balance is itself negative
+ * (ADEsolver_calc.cpp:264-266, 271-273).  They follow the reference text line
+ * by line and are covered only by the GPU-vs-oracle parity tests.
  */
 #ifndef FLUXOS_ORACLE_H
 #define FLUXOS_ORACLE_H

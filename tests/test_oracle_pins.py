@@ -22,8 +22,8 @@ physics and the reference's documented behaviour fix independently:
 * stale fluxes of an all-dry neighbourhood applied again at every step;
 * the ADE's new values within their neighbours' bounds under strong
   transport, and its outflow limiter sending out exactly the held solute
-  through one face, or split a : b between two, and nothing from a cell
-  another face has emptied;
+  through one face, or split a : b between two, nothing from a cell another
+  face has emptied, and from such a cell exactly what it takes in;
 * conservation: water in a walled basin, solute under pure transport;
 * diffusion: one ADE sweep with still water is the explicit 5-point scheme
   c + 2 D dt / dx^2 * Laplacian(c) (sigc = 0.5), exact on a quadratic;
@@ -261,6 +261,33 @@ def check_ade_emptied_cell(lib=None):
     assert c[5, 6] == pytest.approx(cP, rel=1e-12)
     assert abs(c[6, 6]) < 1e-15
     assert c[7, 6] == 0.0
+
+
+def check_ade_pass_through(lib=None):
+    """Still water (no diffusion); a cell P holding solute is emptied through
+    its west face (or its south face), takes solute in through its north face
+    (or its east face) from a neighbour holding some, and its discharge through
+    the remaining face points out: P can pass on only what it takes in
+    (ADEsolver_calc.cpp:259-273).  So P ends at zero, the neighbour that
+    emptied it at P's concentration, and at equal depths the cell downstream of
+    P gains exactly what the supplying neighbour lost."""
+    n, cP, cS = 14, 0.5, 0.4
+    for axis in (0, 1):
+        d = _ade_box(lib, lambda i, j: 0.0 * i, D=0.0, n=n)
+        d.conc[0][6, 6] = cP
+        if axis == 0:      # emptied west, supplied from the north, sends east
+            drain, supply, out = (5, 6), (6, 7), (7, 6)
+            d.fe1[5, 6], d.fn1[6, 6], d.fe1[6, 6] = -0.5, -0.05, 0.3
+        else:              # emptied south, supplied from the east, sends north
+            drain, supply, out = (6, 5), (7, 6), (6, 7)
+            d.fn1[6, 5], d.fe1[6, 6], d.fn1[6, 6] = -0.5, -0.05, 0.3
+        d.conc[0][supply] = cS
+        d.ldry_prev[...] = 1.0                 # no rescale: depths do not change
+        d.adesolver(2, 0)
+        c = d.conc[0]
+        assert abs(c[6, 6]) < 1e-15, axis
+        assert c[drain] == pytest.approx(cP, rel=1e-12), axis
+        assert 0.0 < c[out] == pytest.approx(cS - c[supply], rel=1e-12), (axis, c[out], cS - c[supply])
 
 
 def check_ade_clamp(lib=None):
@@ -898,3 +925,7 @@ def test_ade_outflow_limiter_single_face():
 
 def test_ade_emptied_cell_sends_nothing_on():
     check_ade_emptied_cell()
+
+
+def test_ade_emptied_cell_passes_on_what_it_takes_in():
+    check_ade_pass_through()
