@@ -744,7 +744,9 @@ void orc_adesolver(orc_domain *d, int it, int ichem, double *scratch)
 #else
                 if (cbilan > 0.) pfe = fmin(pfe, cbilan);
 #endif
+#ifndef ORC_PLANT_ADE_NEG
                 else pfe = 0.;
+#endif
             } else if (pfe < 0. && qfn >= 0.) {
                 double cbilan = cvolrat - pfe;
                 if (cbilan > 0.) qfn = fmin(qfn, cbilan);

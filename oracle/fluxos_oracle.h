@@ -66,10 +66,11 @@
  *                      north), or split a : b between two; nothing from a
  *                      cell another face has emptied, and from such a cell
  *                      exactly what it takes in through a third face
- * Parity unpinned (no closed form and no reference vectors): the two
- * sub-branches where an emptied cell's balance is itself negative
- * (ADEsolver_calc.cpp:264-266, 271-273).  They follow the reference text line
- * by line and are covered only by the GPU-vs-oracle parity tests.
+ *                      (nothing, if it was drained of more than it holds)
+ * Parity unpinned: the weir discharge sqrt(g) h^1.5 of outer-ring cells that
+ * are not weir cells (solver_drydomain.cpp:178-185, solver_wetdomain.cpp:
+ * 300-307).  Supported inputs never reach it: the library refuses a domain
+ * whose outer interior ring is not all weir cells.
  */
 #ifndef FLUXOS_ORACLE_H
 #define FLUXOS_ORACLE_H

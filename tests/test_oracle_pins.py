@@ -23,7 +23,8 @@ physics and the reference's documented behaviour fix independently:
 * the ADE's new values within their neighbours' bounds under strong
   transport, and its outflow limiter sending out exactly the held solute
   through one face, or split a : b between two, nothing from a cell another
-  face has emptied, and from such a cell exactly what it takes in;
+  face has emptied, and from such a cell exactly what it takes in (nothing,
+  if it was drained of more than it holds);
 * conservation: water in a walled basin, solute under pure transport;
 * diffusion: one ADE sweep with still water is the explicit 5-point scheme
   c + 2 D dt / dx^2 * Laplacian(c) (sigc = 0.5), exact on a quadratic;
@@ -288,6 +289,31 @@ def check_ade_pass_through(lib=None):
         assert abs(c[6, 6]) < 1e-15, axis
         assert c[drain] == pytest.approx(cP, rel=1e-12), axis
         assert 0.0 < c[out] == pytest.approx(cS - c[supply], rel=1e-12), (axis, c[out], cS - c[supply])
+
+
+def check_ade_overdrained_cell(lib=None):
+    """Still water (no diffusion); a cell P drained through both its west and
+    its south face, each taking all that P holds (each face's flux is capped
+    at P's content), so P's balance is negative; a smaller inflow through its
+    north face (or its east face) does not make up for it, and P's discharge
+    through the remaining face points out: P sends nothing through that face
+    (ADEsolver_calc.cpp:264-266, 271-273), so the cell downstream of it keeps
+    its zero."""
+    n, cP, cS = 14, 0.5, 0.4
+    for axis in (0, 1):
+        d = _ade_box(lib, lambda i, j: 0.0 * i, D=0.0, n=n)
+        d.conc[0][6, 6] = cP
+        d.fe1[5, 6], d.fn1[6, 5] = -0.5, -0.5          # west and south faces drain P
+        if axis == 0:      # supplied from the north, sends east
+            supply, out = (6, 7), (7, 6)
+            d.fn1[6, 6], d.fe1[6, 6] = -0.05, 0.3
+        else:              # supplied from the east, sends north
+            supply, out = (7, 6), (6, 7)
+            d.fe1[6, 6], d.fn1[6, 6] = -0.05, 0.3
+        d.conc[0][supply] = cS
+        d.ldry_prev[...] = 1.0                 # no rescale: depths do not change
+        d.adesolver(2, 0)
+        assert d.conc[0][out] == 0.0, axis
 
 
 def check_ade_clamp(lib=None):
@@ -929,3 +955,7 @@ def test_ade_emptied_cell_sends_nothing_on():
 
 def test_ade_emptied_cell_passes_on_what_it_takes_in():
     check_ade_pass_through()
+
+
+def test_ade_overdrained_cell_sends_nothing_on():
+    check_ade_overdrained_cell()

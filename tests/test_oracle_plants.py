@@ -40,6 +40,7 @@ PLANTS = [
     ("ADE_LIMIT1", pins.check_ade_single_outflow),  # outflow of a single face not limited
     ("ADE_BILAN", pins.check_ade_emptied_cell),   # outflows of an emptied cell not cancelled
     ("ADE_PASS", pins.check_ade_pass_through),    # an emptied cell's east outflow not limited to its inflow
+    ("ADE_NEG", pins.check_ade_overdrained_cell),  # an overdrained cell's east outflow not cancelled
 ]
 
 
