@@ -439,6 +439,15 @@ __device__ inline void update_col(const sweep_ctx &X, ring &R, int c, double &dt
     if (!X.gate) {
         qxn = R.qx[c & 1][i];
         qyn = R.qy[c & 1][i];
+        // no solver step, so every cell was at or below hdry at the CFL block,
+        // which marks it dry (main.cpp:300-325); add_meteo, when there is a
+        // melt table, then marks wet each cell its melt leaves at or below
+        // hdry (common.cpp:219-222), and add_inflow sets its cell from its own
+        // depth (common.cpp:333-337)
+        const double hm = fmax(0.0, d);
+        const bool dry = (p == X.inflow_p) ? hm <= P.hdry
+                                           : (!P.meteo || (f & FXF_NODATA) || hm > P.hdry);
+        nf = dry ? (uint8_t)(nf | FXF_DRY4) : (uint8_t)(nf & ~FXF_DRY4);
     } else {
         const float kf = X.kf, dtf = X.dtf;
         const float fe1W = rflux(R, fs, 0, i - 1), fn1S = rflux(R, fsS, 3, i);
@@ -454,6 +463,8 @@ __device__ inline void update_col(const sweep_ctx &X, ring &R, int c, double &dt
             if (R.fl[SN][i] & LX) wv -= (double)e1;
         }
         const double hd = fmax(0.0, d);
+        // the reference's loop-4 wet flag (hydrodynamics_calc.cpp:123-154)
+        nf = (hd < P.hdry || weir) ? (uint8_t)(nf | FXF_DRY4) : (uint8_t)(nf & ~FXF_DRY4);
         if (hd < P.hdry || weir) {
             qxn = 0.f;
             qyn = 0.f;
@@ -1028,7 +1039,7 @@ int fx32_set_field(fx_domain d, int field, const void *host)
         for (int icol = 1; icol <= d->desc.ncols; icol++)
             for (int irow = 1; irow <= d->desc.nrows; irow++) {
                 const int64_t p = irow + (int64_t)icol * d->mrows;
-                uint8_t f = FXF_INNER;
+                uint8_t f = FXF_INNER | FXF_DRY4;
                 if (w[p]) f |= FXF_WEIR;
                 if (d->zb_host[p] == nod) f |= FXF_NODATA;
                 fl[p] = f;
@@ -1080,7 +1091,18 @@ int fx32_set_field(fx_domain d, int field, const void *host)
         FX_HIP(hipMemcpy(t.flags, fl.data(), mm, hipMemcpyHostToDevice));
         return FX_OK;
     }
-    case FX_F_H: case FX_F_LDRY: case FX_F_LDRY_PREV: case FX_F_H0:
+    case FX_F_LDRY: {                // the flag bit the solver steps keep
+        const float *ld = (const float *)host;
+        std::vector<uint8_t> fl(mm);
+        FX_HIP(hipMemcpy(fl.data(), t.flags, mm, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < mm; i++) {
+            if (!(fl[i] & FXF_INNER)) continue;
+            fl[i] = (uint8_t)(ld[i] == 1.0f ? (fl[i] | FXF_DRY4) : (fl[i] & ~FXF_DRY4));
+        }
+        FX_HIP(hipMemcpy(t.flags, fl.data(), mm, hipMemcpyHostToDevice));
+        return FX_OK;
+    }
+    case FX_F_H: case FX_F_LDRY_PREV: case FX_F_H0:
         return FX_OK;                // derived from d in this state form
     default:
         if (field >= FX_F_CONC0 && field < FX_F_CONC0 + d->desc.nchem) {
@@ -1137,7 +1159,14 @@ int fx32_get_field(fx_domain d, int field, void *host)
         for (int64_t i = 0; i < mm; i++) out[i] = q[i];
         return FX_OK;
     }
-    case FX_F_LDRY: case FX_F_LDRY_PREV: {
+    case FX_F_LDRY: {                // as the last solver step left it
+        if ((rc = load_fl())) return rc;
+        float *o = (float *)host;
+        for (int64_t i = 0; i < mm; i++)
+            o[i] = (fl[i] & FXF_INNER) && (fl[i] & FXF_DRY4) ? 1.0f : 0.0f;
+        return FX_OK;
+    }
+    case FX_F_LDRY_PREV: {           // derived from the current depth
         if ((rc = load_d())) return rc;
         if ((rc = load_fl())) return rc;
         float *o = (float *)host;

@@ -89,6 +89,7 @@ struct fx_state32 {
 #define FXF_NODATA  4u   // bed equals NODATA: no melt
 #define FXF_WETPREV 8u   // wet in loop 1 of the last solver step (open wet-time run)
 #define FXF_INNER  16u   // interior cell (1..NROWS x 1..NCOLS)
+#define FXF_DRY4   32u   // dry at the end of the last solver step: the reference's ldry
 
 // per-kernel HIP-event timing on the domain's stream (fx_profile)
 struct fx_prof {

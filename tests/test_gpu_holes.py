@@ -74,8 +74,8 @@ def test_fp64_holes_and_evaporation_bitwise():
 
 def test_fp32_holes_and_evaporation():
     """FP32-mixed from the same start: depth within 1e-3 m of FP64 shortly
-    after the basin wets, and its FP64 water ledger (negative melt included)
-    closing to 1e-9 over the 30 minutes."""
+    after the basin wets, the same wet flags, and its FP64 water ledger
+    (negative melt included) closing to 1e-9 over the 30 minutes."""
     sc = _scenario()
     init = capi.prepare(sc)
     d64 = capi.HipDomain.from_scenario(sc, precision=capi.FP64_PARITY, init=init)
@@ -90,9 +90,13 @@ def test_fp32_holes_and_evaporation():
                 pn += 300.0
     h64, h32 = d64.get("h")[1:-1, 1:-1], d32.get("h")[1:-1, 1:-1]
     err = float(np.abs(h64 - h32).max())
-    _say(f"t={d64.clock()[0]:.3f}/{d32.clock()[0]:.3f} max|dh| {err:.3e} m, wet "
-         f"{int((d64.get('ldry')[1:-1, 1:-1] == 0).sum())}/{int((d32.get('ldry')[1:-1, 1:-1] == 0).sum())}")
+    w64 = int((d64.get("ldry")[1:-1, 1:-1] == 0).sum())
+    w32 = int((d32.get("ldry")[1:-1, 1:-1] == 0).sum())
+    _say(f"t={d64.clock()[0]:.3f}/{d32.clock()[0]:.3f} max|dh| {err:.3e} m, wet {w64}/{w32}")
     assert err <= 1e-3
+    # every cell sits just below hdry and the solver gate is closed: both
+    # paths report the wet flag the last solver step left (all wet)
+    assert w64 == w32 == 64 * 64
     capi.run_loop(d32, sc.t_end, sc.print_step, timstart=700)
     v, melt, weir = d32.mass_balance()
     rel = abs(v - v0 - melt - weir) / max(abs(v), 1e-300)
