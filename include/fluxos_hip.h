@@ -64,7 +64,11 @@ typedef struct fx_domain_desc {
     int32_t device;              /* HIP device ordinal                        */
 } fx_domain_desc;
 
-/* state fields for fx_set_field / fx_get_field (GlobVar.h:81-92) */
+/* state fields for fx_set_field / fx_get_field (GlobVar.h:81-92).  An
+ * FX_FP32_MIXED domain keeps d = z - zb, qx, qy, the stored fluxes, the
+ * wet-time tracer, ldry (as the reference sets it) and the solutes; it derives
+ * h, h0 and ldry_prev from the current depth on reading and ignores them on
+ * writing. */
 #define FX_F_Z          0   /* double  water surface elevation z          */
 #define FX_F_H          1   /* double  depth h                             */
 #define FX_F_QX         2   /* double  qx                                  */
