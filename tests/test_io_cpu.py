@@ -2,9 +2,11 @@
 fx_write_results) and the command-line driver's input handling, on the CPU.
 
 Formats follow read_functions.cpp:112-326, common.cpp:110-161 and
-write_results.cpp:30-102.  The reference ships no fixtures for them, so the
-checks use files written by this package's generator and hand-made edge
-cases (header order, integer CELLSIZE, the unsigned-long print threshold)."""
+write_results.cpp:30-102.  The reference ships no tests for them and one
+input file (its Working_example melt forcing, kept under tests/golden/), so
+the checks use that file, files written by this package's generator and
+hand-made edge cases (header order, integer CELLSIZE, the unsigned-long print
+threshold)."""
 from __future__ import annotations
 
 import ctypes as C
@@ -19,6 +21,7 @@ from fluxos_cpp_amd import build, capi, gen
 
 
 DemHeader, ResultsGrids = capi.DemHeader, capi.ResultsGrids
+GOLDEN = Path(__file__).resolve().parent / "golden"
 
 
 @pytest.fixture(scope="module")
@@ -80,6 +83,29 @@ def test_forcing_csv(lib, tmp_path):
     small = np.zeros((2, 4), dtype=np.float32)
     assert lib.fx_read_forcing(path, 2, 4, small.ctypes.data_as(C.POINTER(C.c_float)),
                                C.byref(n), C.byref(w)) == -1
+
+
+def test_forcing_reference_working_example(lib):
+    """The reference's own melt file (Working_example/Qmelt_info_2009): the
+    first line is skipped as the header even though it holds data, as
+    read_meteo does by starting at row 1 (read_functions.cpp:258); each value
+    is the float nearest the decimal text (the meteo table is FP32,
+    GlobVar.h:72-73); the two-column layout gives two columns."""
+    path = GOLDEN / "Qmelt_info_2009"
+    lines = path.read_text().splitlines()
+    assert len(lines) == 404 and lines[0].startswith("0,")
+    tab = capi.read_forcing_file(path)
+    assert tab.shape == (403, 2)
+    assert tab[0, 0] == 1800.0 and tab[-1, 0] == 725400.0
+    assert tab[0, 1] == np.float32(0.151000252983975)       # second line of the file
+    assert tab[-1, 1] == np.float32(0.730419896333635)      # last line
+    assert set(np.diff(tab[:, 0]).tolist()) == {1799.0, 1800.0, 1801.0}
+    assert tab[:, 1].min() == 0.0 and 89.0 < tab[:, 1].max() < 89.5
+    for i in (0, 57, 211, 402):                             # exact text -> float32, line by line
+        t, r = lines[i + 1].split(",")
+        assert tab[i, 0] == np.float32(float(t)) and tab[i, 1] == np.float32(float(r))
+    wide = capi.read_forcing_file(path, ncols=3)             # a concentration column the file lacks
+    assert np.array_equal(wide[:, :2], tab) and not wide[:, 2].any()
 
 
 def _dp(a):

@@ -1,8 +1,9 @@
 """Pins of the CPU oracle that do not restate its formulas.
 
-The reference ships no tests, fixtures or golden vectors, and it cannot be
-built here (Armadillo is absent), so the oracle is pinned against what the
-physics and the reference's documented behaviour fix independently:
+The reference ships no tests or golden vectors, only one input file (a melt
+record, used by the reader and device tests), and it cannot be built here
+(Armadillo is absent), so the oracle is pinned against what the physics and
+the reference's documented behaviour fix independently:
 
 * CFL: dt = cfl*dxy/(max(|u|,|v|,1e-6) + sqrt(g h)) -- textbook bound;
 * forcing: the "first row later than t" lookup, unit conversion mm/day -> m;
