@@ -93,7 +93,7 @@ fx_params fx_make_params(fx_domain d)
     P.inflow_nrows = d->inflow_nrows;
     P.inflow_ncols = d->inflow_ncols;
     P.inflow_p = d->inflow ? d->inflow_irow + d->inflow_icol * (int64_t)d->mrows : -1;
-    P.rates_nonneg = d->rates_nonneg ? 1 : 0;
+    P.rates_nonneg = (d->meteo_nonneg && d->inflow_nonneg) ? 1 : 0;
     P.ch = d->ch;
     P.ach = d->ach;
     P.wintra = d->wintra ? 1 : 0;
@@ -195,6 +195,25 @@ int enter(fx_domain d)
     if (!d) return fx_fail(FX_ERR_ARG, "domain is NULL");
     FX_HIP(hipSetDevice(d->desc.device));
     return FX_OK;
+}
+
+// Entry points that change a domain's state, clock or tables.  A batch member
+// belongs to its batch, whose slot holds the member's launch parameters and
+// table addresses from admission on: it is refused until evicted.
+int enter_alone(fx_domain d)
+{
+    int rc = enter(d);
+    if (rc) return rc;
+    if (d->in_batch) return fx_fail(FX_ERR_STATE, "domain is a batch member: evict it first");
+    return FX_OK;
+}
+
+// The captured loop iteration holds table addresses and launch parameters by
+// value: drop it before anything it names is freed or changed.
+void drop_graph(fx_domain d)
+{
+    if (d->graph_ok) (void)hipGraphExecDestroy(d->graph);
+    d->graph_ok = false;
 }
 
 int pull_ctl(fx_domain d)
@@ -315,7 +334,8 @@ int fx_domain_create(const fx_domain_desc *desc, fx_domain *out)
     d->mrows = desc->nrows + 2;
     d->mcols = desc->ncols + 2;
     d->ncell = (int64_t)d->mrows * d->mcols;
-    d->rates_nonneg = true;
+    d->meteo_nonneg = true;
+    d->inflow_nonneg = true;
     d->graph_ok = false;
     d->ch = FX32_CH;
     d->ach = FX32_ACH;
@@ -394,7 +414,7 @@ int fx_domain_destroy(fx_domain d)
 
 int fx_set_static(fx_domain d, const double *zb, const double *ks, const float *weir)
 {
-    int rc = enter(d);
+    int rc = enter_alone(d);
     if (rc) return rc;
     if (!zb || !ks || !weir) return fx_fail(FX_ERR_ARG, "zb, ks and weir are required");
     const int64_t mm = d->ncell;
@@ -430,7 +450,7 @@ int fx_set_static(fx_domain d, const double *zb, const double *ks, const float *
 
 int fx_set_field(fx_domain d, int32_t field, const void *host)
 {
-    int rc = enter(d);
+    int rc = enter_alone(d);
     if (rc) return rc;
     if (!host) return fx_fail(FX_ERR_ARG, "host buffer is NULL");
     const int64_t mm = d->ncell;
@@ -513,13 +533,13 @@ int fx_set_wintra(fx_domain d, int32_t status, double soil_release_rate)
     }
     d->wintra = status != 0;
     d->soil_rate = soil_release_rate;
-    if (d->graph_ok) { (void)hipGraphExecDestroy(d->graph); d->graph_ok = false; }
+    drop_graph(d);
     return FX_OK;
 }
 
 int fx_wintrasolver_calc(fx_domain d, int32_t ichem)
 {
-    int rc = enter(d);
+    int rc = enter_alone(d);
     if (rc) return rc;
     if (d->desc.precision != FX_FP64_PARITY)
         return fx_fail(FX_ERR_UNSUPPORTED, "granular calls need an FP64 parity domain");
@@ -532,7 +552,7 @@ int fx_wintrasolver_calc(fx_domain d, int32_t ichem)
 
 int fx_set_clock(fx_domain d, double tim, double dtfl, int32_t it)
 {
-    int rc = enter(d);
+    int rc = enter_alone(d);
     if (rc) return rc;
     if ((rc = pull_ctl(d))) return rc;
     d->ctl_host->tim = tim;
@@ -556,49 +576,59 @@ int fx_get_clock(fx_domain d, double *tim, double *dtfl, int32_t *it)
 
 int fx_set_meteo(fx_domain d, int32_t nrows, int32_t ncols, const float *table, const float *melt_mult)
 {
-    int rc = enter(d);
+    int rc = enter_alone(d);
     if (rc) return rc;
+    const bool has_table = table && nrows > 0;
+    if (has_table && ncols < 2) return fx_fail(FX_ERR_ARG, "meteo table needs at least 2 columns");
+    if (has_table && nrows > 1 << 20) return fx_fail(FX_ERR_ARG, "meteo table too long");
     FX_HIP(hipStreamSynchronize(d->stream));
+    drop_graph(d);
     dfree(d->meteo);
     d->meteo = nullptr;
     d->meteo_nrows = 0;
-    if (table && nrows > 0) {
-        if (ncols < 2) return fx_fail(FX_ERR_ARG, "meteo table needs at least 2 columns");
-        if (nrows > 1 << 20) return fx_fail(FX_ERR_ARG, "meteo table too long");
+    dfree(d->melt_mult);
+    d->melt_mult = nullptr;
+    d->melt_area_ok = false;
+    d->meteo_nonneg = true;
+    if (has_table) {
         for (int32_t r = 0; r < nrows; r++)
-            if (table[(size_t)r * ncols + 1] < 0.0f) d->rates_nonneg = false;
+            if (!(table[(size_t)r * ncols + 1] >= 0.0f)) d->meteo_nonneg = false;
         FX_HIP(hipMalloc((void **)&d->meteo, (size_t)nrows * ncols * sizeof(float)));
         FX_HIP(hipMemcpy(d->meteo, table, (size_t)nrows * ncols * sizeof(float), hipMemcpyHostToDevice));
         d->meteo_nrows = nrows;
         d->meteo_ncols = ncols;
     }
-    dfree(d->melt_mult);
-    d->melt_mult = nullptr;
     if (melt_mult) {
         for (int64_t i = 0; i < d->ncell; i++)
-            if (!(melt_mult[i] >= 0.0f)) { d->rates_nonneg = false; break; }
+            if (!(melt_mult[i] >= 0.0f)) { d->meteo_nonneg = false; break; }
         FX_HIP(hipMalloc((void **)&d->melt_mult, d->ncell * sizeof(float)));
         FX_HIP(hipMemcpy(d->melt_mult, melt_mult, d->ncell * sizeof(float), hipMemcpyHostToDevice));
     }
-    d->melt_area_ok = false;
-    if (d->graph_ok) { (void)hipGraphExecDestroy(d->graph); d->graph_ok = false; }
     return FX_OK;
 }
 
 int fx_set_inflow(fx_domain d, int32_t nrows, int32_t ncols, const float *table, int64_t irow, int64_t icol)
 {
-    int rc = enter(d);
+    int rc = enter_alone(d);
     if (rc) return rc;
-    FX_HIP(hipStreamSynchronize(d->stream));
-    dfree(d->inflow);
-    d->inflow = nullptr;
-    d->inflow_nrows = 0;
-    if (table && nrows > 0) {
+    const bool has_table = table && nrows > 0;
+    if (has_table) {
         if (ncols < 2) return fx_fail(FX_ERR_ARG, "inflow table needs at least 2 columns");
+        if (nrows > 1 << 20) return fx_fail(FX_ERR_ARG, "inflow table too long");
         if (irow < 0 || irow >= d->mrows || icol < 0 || icol >= d->mcols)
             return fx_fail(FX_ERR_ARG, "inflow cell outside the grid");
         if (irow < 1 || irow > d->desc.nrows || icol < 1 || icol > d->desc.ncols)
             return fx_fail(FX_ERR_UNSUPPORTED, "inflow cell on the ghost ring: no interior cell receives it");
+    }
+    FX_HIP(hipStreamSynchronize(d->stream));
+    drop_graph(d);
+    dfree(d->inflow);
+    d->inflow = nullptr;
+    d->inflow_nrows = 0;
+    d->inflow_nonneg = true;
+    if (has_table) {
+        for (int32_t r = 0; r < nrows; r++)
+            if (!(table[(size_t)r * ncols + 1] >= 0.0f)) d->inflow_nonneg = false;
         FX_HIP(hipMalloc((void **)&d->inflow, (size_t)nrows * ncols * sizeof(float)));
         FX_HIP(hipMemcpy(d->inflow, table, (size_t)nrows * ncols * sizeof(float), hipMemcpyHostToDevice));
         d->inflow_nrows = nrows;
@@ -606,13 +636,12 @@ int fx_set_inflow(fx_domain d, int32_t nrows, int32_t ncols, const float *table,
         d->inflow_irow = irow;
         d->inflow_icol = icol;
     }
-    if (d->graph_ok) { (void)hipGraphExecDestroy(d->graph); d->graph_ok = false; }
     return FX_OK;
 }
 
 int fx_courant_condition(fx_domain d, double print_next, double *dtfl, double *hpall)
 {
-    int rc = enter(d);
+    int rc = enter_alone(d);
     if (rc) return rc;
     if (d->desc.precision != FX_FP64_PARITY)
         return fx_fail(FX_ERR_UNSUPPORTED, "granular calls need an FP64 parity domain");
@@ -628,7 +657,7 @@ int fx_courant_condition(fx_domain d, double print_next, double *dtfl, double *h
 
 int fx_add_meteo(fx_domain d)
 {
-    int rc = enter(d);
+    int rc = enter_alone(d);
     if (rc) return rc;
     if (d->desc.precision != FX_FP64_PARITY)
         return fx_fail(FX_ERR_UNSUPPORTED, "granular calls need an FP64 parity domain");
@@ -640,7 +669,7 @@ int fx_add_meteo(fx_domain d)
 
 int fx_add_inflow(fx_domain d)
 {
-    int rc = enter(d);
+    int rc = enter_alone(d);
     if (rc) return rc;
     if (d->desc.precision != FX_FP64_PARITY)
         return fx_fail(FX_ERR_UNSUPPORTED, "granular calls need an FP64 parity domain");
@@ -658,7 +687,7 @@ int fx_add_inflow(fx_domain d)
 
 int fx_hydrodynamics_calc(fx_domain d)
 {
-    int rc = enter(d);
+    int rc = enter_alone(d);
     if (rc) return rc;
     if (d->desc.precision != FX_FP64_PARITY)
         return fx_fail(FX_ERR_UNSUPPORTED, "granular calls need an FP64 parity domain");
@@ -672,7 +701,7 @@ int fx_hydrodynamics_calc(fx_domain d)
 
 int fx_adesolver_calc(fx_domain d, int32_t it, int32_t ichem)
 {
-    int rc = enter(d);
+    int rc = enter_alone(d);
     if (rc) return rc;
     if (d->desc.precision != FX_FP64_PARITY)
         return fx_fail(FX_ERR_UNSUPPORTED, "granular calls need an FP64 parity domain");
@@ -687,7 +716,7 @@ int fx_adesolver_calc(fx_domain d, int32_t it, int32_t ichem)
 
 int fx_advance(fx_domain d, double t_end, double print_next, int32_t max_steps, fx_step_info *info)
 {
-    int rc = enter(d);
+    int rc = enter_alone(d);
     if (rc) return rc;
     if (!d->has_static) return fx_fail(FX_ERR_STATE, "fx_set_static has not been called");
     if (max_steps <= 0) return fx_fail(FX_ERR_ARG, "max_steps must be positive");

@@ -122,7 +122,8 @@ struct fx_domain_s {
     bool has_static;
     double ks0;                // interior roughness (uniform in FP32-mixed domains)
     bool melt_area_ok;         // ctl->melt_area is current
-    bool rates_nonneg;
+    bool meteo_nonneg;         // no negative melt rate or multiplier in the current tables
+    bool inflow_nonneg;        // no negative rate in the current inflow table
     fx_state64 s64;
     fx_state32 s32;
     std::vector<double> zb_host;   // FP64 bed (FP32 mode output conversion)

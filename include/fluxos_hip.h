@@ -202,8 +202,11 @@ int fx_synchronize(fx_domain dom);
  * when its clock reaches its print time or passes t_end; fx_batch_poll
  * reports it, the host may then read its fields with fx_get_field and resume
  * or evict it.  Between fx_batch_run and the next fx_batch_poll the members'
- * state belongs to the batch: no other call may touch a member domain, and a
- * member must be evicted before fx_domain_destroy.                          */
+ * state belongs to the batch: no other call may touch a member domain.  At
+ * any time, calls that change a member's state, clock, tables, tiling or
+ * WINTRA setting, the granular seams, fx_advance and fx_domain_destroy
+ * return FX_ERR_STATE until it is evicted (the batch holds the member's
+ * launch parameters and table addresses from admission on).                 */
 typedef struct fx_batch_s *fx_batch;
 #define FX_EV_PRINT 1       /* member reached its print time (tim <= t_end)   */
 #define FX_EV_DONE  2       /* member's clock passed t_end (main.cpp:271)     */

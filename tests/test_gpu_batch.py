@@ -155,3 +155,42 @@ def test_tiling_does_not_change_results():
                 assert np.allclose(st[k], states[0][k], rtol=1e-12, atol=1e-12)
             else:
                 assert np.array_equal(st[k], states[0][k], equal_nan=True), k
+
+
+def test_members_refuse_changes_until_evicted():
+    """The batch's slot holds a member's launch parameters and table
+    addresses from admission on, so every call that would change the member
+    returns FX_ERR_STATE until it is evicted; reads still work.  The refused
+    calls leave the member's run bit-identical to its alone run."""
+    t_end = 1300.0
+    sc = _scen(48, 21, 45.0)
+    ref = _state(_alone(sc, t_end)[0])
+    d = capi.HipDomain.from_scenario(sc, precision=capi.FP32_MIXED, ade=True)
+    batch = capi.Batch(nchem=1, ade=True, max_members=4)
+    s = batch.admit(d, t_end, PRINT)
+    refused = {
+        "set_meteo": lambda: d.set_meteo(sc.meteo),
+        "set_inflow": lambda: d.set_inflow(sc.meteo, 5, 5),
+        "set_field": lambda: d.set("h", np.zeros((sc.nrows + 2, sc.ncols + 2))),
+        "set_clock": lambda: d.set_clock(0.0),
+        "courant": lambda: d.courant(PRINT),
+        "hydrodynamics": lambda: d.hydrodynamics(),
+        "advance": lambda: d.advance(t_end, PRINT),
+        "set_tiling": lambda: d.set_tiling(8, 16),
+    }
+    for name, call in refused.items():
+        with pytest.raises(capi.FxError) as e:
+            call()
+        assert e.value.code == -3, name
+    assert capi.load().fx_domain_destroy(d.h) == -3
+    d.get("h"), d.clock(), d.mass_balance()                     # reads stay allowed
+    next_print = {s: PRINT}
+    _drive(batch, next_print, set())
+    got = _state(d)
+    for k in ref:
+        assert np.array_equal(ref[k], got[k], equal_nan=True), k
+    batch.evict(s)
+    d.set_clock(5.0, 1.0, 3)                                    # its own again
+    assert d.clock()[:2] == (5.0, 1.0)
+    batch.close()
+    d.close()
