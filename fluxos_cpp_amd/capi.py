@@ -331,8 +331,9 @@ class HipDomain:
         self.h = h
 
     def close(self) -> None:
+        """Free the domain; a batch member is refused (FX_ERR_STATE) and kept."""
         if self.h:
-            self.lib.fx_domain_destroy(self.h)
+            _check(self.lib.fx_domain_destroy(self.h))
             self.h = C.c_void_p()
 
     def __del__(self):
@@ -548,9 +549,11 @@ class Batch:
         self.members: dict[int, HipDomain] = {}
 
     def close(self) -> None:
+        """Free the batch; members still in it leave it with their state."""
         if self.h:
             load().fx_batch_destroy(self.h)
             self.h = None
+            self.members.clear()
 
     def __del__(self):
         try:

@@ -209,6 +209,8 @@ int fx_batch_destroy(fx_batch b)
     if (!b) return FX_OK;
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (fx_domain m : b->member)          // members still in the batch leave it with their state
+        if (m) m->in_batch = false;
     drop_graph(b);
     if (b->slots_dev) (void)hipFree(b->slots_dev);
     if (b->stat_dev) (void)hipFree(b->stat_dev);

@@ -221,6 +221,8 @@ typedef struct fx_batch_event {
     int32_t pad;
 } fx_batch_event;
 int fx_batch_create(int32_t device, int32_t nchem, int32_t ade_solver, int32_t max_members, fx_batch *out);
+/* Waits for the batch's queued steps; members still in it leave it with
+ * their state, as if evicted.                                             */
 int fx_batch_destroy(fx_batch batch);
 /* Admit a prepared domain (static fields, state and forcing set) that runs
  * from its current clock until tim > t_end, halting at print_next.        */

@@ -192,5 +192,10 @@ def test_members_refuse_changes_until_evicted():
     batch.evict(s)
     d.set_clock(5.0, 1.0, 3)                                    # its own again
     assert d.clock()[:2] == (5.0, 1.0)
+    # destroying a batch releases the members still in it
+    s = batch.admit(d, t_end, PRINT)
+    with pytest.raises(capi.FxError):
+        d.close()
     batch.close()
+    d.set_clock(0.0)
     d.close()
