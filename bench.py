@@ -401,7 +401,8 @@ def main() -> int:
         v32w = cells * info_w.solver_steps / el_w
         fp64 = {"value": round(v64, 1), "ms_per_step": round(el64 * 1e3 / max(1, info64.steps), 3),
                 "steps": info64.steps, "window_from_t": round(snap["clock"][0], 1),
-                "fp32_value_same_window": round(v32w, 1), "fp32_over_fp64": round(v32w / v64, 1)}
+                "fp32_value_same_window": round(v32w, 1),
+                "fp32_over_fp64": round(v32w / v64, 1) if v64 > 0 else None}   # None: window still dry
         d64.close()
         d64 = None
         err = accuracy(sc, init, snap, args.accuracy_spans)
@@ -434,6 +435,10 @@ def main() -> int:
             if rate <= 0:
                 continue
             reqs = serve.c4_stream(rate=rate, seconds=args.serve_seconds)
+            if not reqs:                                # a short window can draw no arrival
+                serving[label] = {"requests": 0, "arrival_rate_per_s": rate,
+                                  "arrival_window_s": args.serve_seconds}
+                continue
             r = serve.serve(reqs)
             r["arrival_rate_per_s"] = rate
             r["arrival_window_s"] = args.serve_seconds
