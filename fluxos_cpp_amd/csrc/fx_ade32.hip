@@ -207,7 +207,7 @@ __device__ inline void a_shift(awin<NC> &W)
 // fluxes of column c (window slot 1); with upd, also the update of the own
 // rows.  Returns nothing: the limited N-face flux is carried to column c+1.
 template <int NC>
-__device__ inline void a_column(const actx &X, awin<NC> &W, int c, bool upd, bool own)
+__device__ inline void a_column(const actx &X, awin<NC> &W, int c, bool upd, bool own, bool &bad)
 {
     const ageo &gP = W.g[1], &gN = W.g[2], &gNN = W.g[3], &gS = W.g[0];
     const bool wW = __shfl_up((int)gP.w, 1, 64), wE = __shfl_down((int)gP.w, 1, 64);
@@ -256,6 +256,7 @@ __device__ inline void a_column(const actx &X, awin<NC> &W, int c, bool upd, boo
                 con = fmaxf(fminf(fminf(cW, crE), fminf(cs, cn)), con);
                 out = con * gP.h;
             }
+            bad |= X.transport && gP.w && !isfinite(out);   // reported as ctl->status bit 2
             at_off(X.m_out[k], p * 4u) = out;
         }
         W.pn_s[k] = pn;
@@ -303,6 +304,7 @@ __device__ inline void ade_tile(const fx_params &P, const a32p &A, fx_ctl *ctl, 
 
     awin<NC> W;
     apref<NC> a, b;
+    bool bad = false;
     // prologue: window = columns c0-3 .. c0; the N-face fluxes of column
     // c0-2 (before limiting) and of column c0-1 (both) feed column c0
     a_load<NC>(X, c0 - 3, a); a_derive<NC>(X, a, W.g[0], W.c1[0], W.cr[0]);
@@ -311,10 +313,10 @@ __device__ inline void ade_tile(const fx_params &P, const a32p &A, fx_ctl *ctl, 
     a_load<NC>(X, c0, a);     a_derive<NC>(X, a, W.g[3], W.c1[3], W.cr[3]);
 #pragma unroll
     for (int k = 0; k < NC; k++) { W.pn_s[k] = 0.f; W.n_s[k] = 0.f; }
-    a_column<NC>(X, W, c0 - 2, false, own);
+    a_column<NC>(X, W, c0 - 2, false, own, bad);
     a_shift<NC>(W);
     a_load<NC>(X, c0 + 1, a); a_derive<NC>(X, a, W.g[3], W.c1[3], W.cr[3]);
-    a_column<NC>(X, W, c0 - 1, false, own);
+    a_column<NC>(X, W, c0 - 1, false, own, bad);
     if constexpr (NC >= 3) {
         // three or four solutes: one column of prefetch (a: c+2) keeps the
         // kernel at five waves per SIMD, which large grids of many rounds need
@@ -323,8 +325,9 @@ __device__ inline void ade_tile(const fx_params &P, const a32p &A, fx_ctl *ctl, 
             a_shift<NC>(W);
             a_derive<NC>(X, a, W.g[3], W.c1[3], W.cr[3]);
             if (c + 3 <= c_last + 2) a_load<NC>(X, c + 3, a);
-            a_column<NC>(X, W, c, true, own);
+            a_column<NC>(X, W, c, true, own, bad);
         }
+        if (__any(bad) && lane == 0) atomicOr(&ctl->status, 4);
         return;
     }
     // steady state: prefetch two columns ahead (a: c+2, b: c+3)
@@ -335,17 +338,18 @@ __device__ inline void ade_tile(const fx_params &P, const a32p &A, fx_ctl *ctl, 
         a_shift<NC>(W);
         a_derive<NC>(X, a, W.g[3], W.c1[3], W.cr[3]);
         if (c + 4 <= c_last + 2) a_load<NC>(X, c + 4, a);
-        a_column<NC>(X, W, c, true, own);
+        a_column<NC>(X, W, c, true, own, bad);
         a_shift<NC>(W);
         a_derive<NC>(X, b, W.g[3], W.c1[3], W.cr[3]);
         if (c + 5 <= c_last + 2) a_load<NC>(X, c + 5, b);
-        a_column<NC>(X, W, c + 1, true, own);
+        a_column<NC>(X, W, c + 1, true, own, bad);
     }
     if (c <= c_last) {
         a_shift<NC>(W);
         a_derive<NC>(X, a, W.g[3], W.c1[3], W.cr[3]);
-        a_column<NC>(X, W, c, true, own);
+        a_column<NC>(X, W, c, true, own, bad);
     }
+    if (__any(bad) && lane == 0) atomicOr(&ctl->status, 4);
 }
 
 template <int NC>

@@ -238,6 +238,9 @@ int fx_batch_admit(fx_batch b, fx_domain d, double t_end, double print_next, int
         return fx_fail(FX_ERR_ARG, "the domain's solutes differ from the batch's");
     for (int s = 0; s < b->maxm; s++)
         if (b->member[s] == d) return fx_fail(FX_ERR_STATE, "domain is already a member");
+    // a domain steps in one batch only: two batches would advance the same
+    // state and control block on different streams
+    if (d->in_batch) return fx_fail(FX_ERR_STATE, "domain is a member of another batch: evict it first");
     int s = 0;
     while (s < b->maxm && b->member[s]) s++;
     if (s == b->maxm) return fx_fail(FX_ERR_STATE, "batch is full");

@@ -421,7 +421,7 @@ __device__ inline void flux_col(const sweep_ctx &X, ring &R, int c)
 // U(c): hydro loops 3 and 4 for this lane's own cell, and its CFL bound
 template <int S0, int SS, int SN>
 __device__ inline void update_col(const sweep_ctx &X, ring &R, int c, double &dtm, double &hmx,
-                                  double &wv)
+                                  double &wv, bool &bad)
 {
     if (!X.own) return;
     const fx_params &P = *X.P;
@@ -499,6 +499,12 @@ __device__ inline void update_col(const sweep_ctx &X, ring &R, int c, double &dt
             }
         }
     }
+    // a non-finite depth or discharge is reported (ctl->status bit 2): the
+    // wet/dry tests below would otherwise read NaN as a dry cell.  Weir cells
+    // are left out: they never compute, and initiation leaves the bed of the
+    // far corner NaN (0/0 of four NODATA vertices, initiate.cpp:85), so its
+    // water surface is NaN in the reference too
+    bad |= !weir && (!isfinite(d) || !isfinite(qxn) || !isfinite(qyn));
     at_off(X.d_out, p * 8u) = d;
     at_off(X.qx_out, o4) = qxn;
     at_off(X.qy_out, o4) = qyn;
@@ -513,7 +519,7 @@ __device__ inline void update_col(const sweep_ctx &X, ring &R, int c, double &dt
 // one column of the sweep; SC = ring slot of column c
 template <int SC>
 __device__ inline void column_step(const sweep_ctx &X, ring &R, int c, int c_last, pref &a,
-                                   double &dtm, double &hmx, double &wv, double &ms)
+                                   double &dtm, double &hmx, double &wv, double &ms, bool &bad)
 {
     constexpr int SN = (SC + 1) % 3, SS = (SC + 2) % 3;
     __syncthreads();
@@ -522,7 +528,7 @@ __device__ inline void column_step(const sweep_ctx &X, ring &R, int c, int c_las
     __syncthreads();
     flux_col<SC, SS, SN>(X, R, c);
     __syncthreads();
-    update_col<SC, SS, SN>(X, R, c, dtm, hmx, wv);
+    update_col<SC, SS, SN>(X, R, c, dtm, hmx, wv, bad);
 }
 
 // one block of the sweep: strip bx (rows), column block by; gx = strips per
@@ -572,6 +578,7 @@ __device__ inline void sweep_tile(const fx_params &P, const s32p &S, fx_ctl *ctl
 
     pref a;
     double dtm = 9.e10, hmx = 0.0, wv = 0.0, ms = 0.0;
+    bool bad = false;
     // prologue: columns c0-2 (slot 1), c0-1 (slot 2), c0 (slot 0), fluxes of c0-1
     load_col(X, c0 - 2, a); store_col<1>(X, R, a, c0 - 2, false, ms);
     load_col(X, c0 - 1, a); store_col<2>(X, R, a, c0 - 1, false, ms);
@@ -582,16 +589,17 @@ __device__ inline void sweep_tile(const fx_params &P, const s32p &S, fx_ctl *ctl
     // column c uses ring slot (c - c0) % 3, unrolled by three
     int c = c0;
     for (; c + 2 <= c_last; c += 3) {
-        column_step<0>(X, R, c, c_last, a, dtm, hmx, wv, ms);
-        column_step<1>(X, R, c + 1, c_last, a, dtm, hmx, wv, ms);
-        column_step<2>(X, R, c + 2, c_last, a, dtm, hmx, wv, ms);
+        column_step<0>(X, R, c, c_last, a, dtm, hmx, wv, ms, bad);
+        column_step<1>(X, R, c + 1, c_last, a, dtm, hmx, wv, ms, bad);
+        column_step<2>(X, R, c + 2, c_last, a, dtm, hmx, wv, ms, bad);
     }
-    if (c <= c_last) { column_step<0>(X, R, c, c_last, a, dtm, hmx, wv, ms); c++; }
-    if (c <= c_last) { column_step<1>(X, R, c, c_last, a, dtm, hmx, wv, ms); c++; }
+    if (c <= c_last) { column_step<0>(X, R, c, c_last, a, dtm, hmx, wv, ms, bad); c++; }
+    if (c <= c_last) { column_step<1>(X, R, c, c_last, a, dtm, hmx, wv, ms, bad); c++; }
     dtm = wmin(dtm);
     hmx = wmax(hmx);
     wv = wsum(wv);
     ms = wsum(ms);
+    if (__any(bad) && X.k == 0) atomicOr(&ctl->status, 4);
     if (X.k == 0) {
         const size_t bid = bx + (size_t)by * gx;
         S.partials[FX32_NPV * bid] = dtm;
@@ -858,7 +866,7 @@ __global__ void k32_flip_end(fx_ctl *ctl)
 {
     if (!ctl->halt) {
         ctl->cur ^= 1;
-        ctl->halt = ctl->pending_halt;
+        ctl->halt = ctl->pending_halt | ((ctl->status >> 2) & 1);     // halt on NaN/Inf
     }
 }
 
@@ -871,7 +879,7 @@ __global__ void k32b_end(const fx_bslot *slots, int nslot, fx_bstat *stat)
     fx_ctl *ctl = slots[i].ctl;
     if (!ctl->halt) {
         ctl->cur ^= 1;
-        ctl->halt = ctl->pending_halt;
+        ctl->halt = ctl->pending_halt | ((ctl->status >> 2) & 1);     // a NaN/Inf member halts alone
     }
     fx_bstat st;
     st.tim = ctl->tim;
@@ -1309,15 +1317,32 @@ int fx32_launch_step(fx_domain d)
     return fx_hip_check(hipGetLastError(), "k32_flip_end");
 }
 
-int fx32_launch_volume(fx_domain d, double *vol_out)
+int64_t fx32_volume_nparts(fx_domain d)
+{
+    const dim3 g = grid32(fx_make_params(d));
+    return (int64_t)g.x * g.y;
+}
+
+// block partials of the water volume into part[0 .. fx32_volume_nparts)
+int fx32_volume_partials(fx_domain d, double *part)
 {
     fx_params P = fx_make_params(d);
     s32p S = make_s32p(d);
-    dim3 g = grid32(P);
-    const size_t nb = (size_t)g.x * g.y;
+    hipLaunchKernelGGL(k32_volume, grid32(P), dim3(64), 0, d->stream, P, S, d->ctl, part);
+    return fx_hip_check(hipGetLastError(), "k32_volume");
+}
+
+int fx32_launch_volume(fx_domain d, double *vol_out)
+{
+    fx_params P = fx_make_params(d);
+    const size_t nb = (size_t)fx32_volume_nparts(d);
     double *part = nullptr;
     FX_HIP(hipMalloc((void **)&part, nb * sizeof(double)));
-    hipLaunchKernelGGL(k32_volume, g, dim3(64), 0, d->stream, P, S, d->ctl, part);
+    int rc = fx32_volume_partials(d, part);
+    if (rc) {
+        (void)hipFree(part);
+        return rc;
+    }
     std::vector<double> h(nb);
     FX_HIP(hipMemcpyAsync(h.data(), part, nb * sizeof(double), hipMemcpyDeviceToHost, d->stream));
     FX_HIP(hipStreamSynchronize(d->stream));

@@ -516,7 +516,7 @@ __global__ void k_forcing(fx_params P, fx_ctl *ctl) { fx_forcing_eval(P, ctl); }
 
 __global__ void k_step_end(fx_ctl *ctl)
 {
-    if (!ctl->halt) ctl->halt = ctl->pending_halt;
+    if (!ctl->halt) ctl->halt = ctl->pending_halt | ((ctl->status >> 2) & 1);   // halt on NaN/Inf
 }
 
 // water volume of non-weir cells (diagnostic, block partials + host sum)
@@ -639,14 +639,31 @@ int fx64_launch_solute(fx_domain d, int k, double *mass_out)
     return FX_OK;
 }
 
+int64_t fx64_volume_nparts(fx_domain d)
+{
+    const dim3 g = grid_of(fx_make_params(d));
+    return (int64_t)g.x * g.y;
+}
+
+// block partials of the water volume into part[0 .. fx64_volume_nparts)
+int fx64_volume_partials(fx_domain d, double *part)
+{
+    fx_params P = fx_make_params(d);
+    hipLaunchKernelGGL(k64_volume, grid_of(P), dim3(BX, BY), 0, d->stream, P, d->s64, part);
+    return fx_hip_check(hipGetLastError(), "k64_volume");
+}
+
 int fx64_launch_volume(fx_domain d, double *vol_out)
 {
     fx_params P = fx_make_params(d);
-    dim3 g = grid_of(P);
-    const size_t nb = (size_t)g.x * g.y;
+    const size_t nb = (size_t)fx64_volume_nparts(d);
     double *part = nullptr;
     FX_HIP(hipMallocAsync((void **)&part, nb * sizeof(double), d->stream));
-    hipLaunchKernelGGL(k64_volume, g, dim3(BX, BY), 0, d->stream, P, d->s64, part);
+    int rc = fx64_volume_partials(d, part);
+    if (rc) {
+        (void)hipFreeAsync(part, d->stream);
+        return rc;
+    }
     std::vector<double> h(nb);
     FX_HIP(hipMemcpyAsync(h.data(), part, nb * sizeof(double), hipMemcpyDeviceToHost, d->stream));
     FX_HIP(hipFreeAsync(part, d->stream));

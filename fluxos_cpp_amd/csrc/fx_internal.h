@@ -170,4 +170,9 @@ int fx32_launch_volume(fx_domain d, double *vol_out);
 int fx32_launch_ade(fx_domain d);
 int fx32_pending_weir(fx_domain d, double *vol);
 int fx32_launch_solute(fx_domain d, int k, double *mass_out);
+// water-volume block partials of fx_mass_balance into a caller's buffer
+int64_t fx32_volume_nparts(fx_domain d);
+int fx32_volume_partials(fx_domain d, double *part);
+int64_t fx64_volume_nparts(fx_domain d);
+int fx64_volume_partials(fx_domain d, double *part);
 int fx64_launch_solute(fx_domain d, int k, double *mass_out);

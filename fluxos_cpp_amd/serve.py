@@ -41,6 +41,8 @@ class Request:
     steps: int = 0
     solver_steps: int = 0
     result: np.ndarray | None = None     # depth at the end time (padded grid)
+    error: str = ""                      # why the request failed ("" = it completed)
+    queued: bool = False                 # waited for memory or a slot after it arrived
 
     @property
     def cells(self) -> int:
@@ -64,13 +66,18 @@ def domain_bytes(sc: gen.Scenario, ade: bool = True) -> int:
 
 
 def _drive(batch: capi.Batch, t_end: dict, next_print: dict, print_step: float, on_done,
-           steps_per_poll: int) -> None:
+           steps_per_poll: int, on_error=None) -> None:
     """One run/poll round: resume members at intermediate prints, hand the
-    finished ones to on_done(slot, tim, steps)."""
+    finished ones to on_done(slot, tim, steps, solver_steps) and members whose
+    state holds NaN/Inf (halted alone by the device) to on_error(slot, tim);
+    without on_error such a member raises."""
     batch.run(steps_per_poll)
     for slot, kind, tim, it, steps, solver_steps in batch.poll():
         if kind == capi.EV_ERROR:
-            raise RuntimeError(f"member {slot}: NaN/Inf in the state")
+            if on_error is None:
+                raise RuntimeError(f"member {slot}: NaN/Inf in the state")
+            on_error(slot, tim)
+            continue
         if kind == capi.EV_DONE or tim >= t_end[slot]:
             on_done(slot, tim, steps, solver_steps)
         else:
@@ -92,20 +99,51 @@ def run_ensemble(scens: list[gen.Scenario], inits: list[capi.InitialState], ade:
         s = batch.admit(d, float(sc.t_end), float(sc.print_step))
         doms[s], t_end[s], next_print[s] = d, float(sc.t_end), float(sc.print_step)
     done: dict[int, int] = {}
-    batch.poll()
-    t0 = time.perf_counter()
-    while len(done) < len(doms):
-        _drive(batch, t_end, next_print, float(scens[0].print_step),
-               lambda s, tim, steps, solver: done.__setitem__(s, steps), steps_per_poll)
-    el = time.perf_counter() - t0
-    n, cells, upd = batch.info()
+    failed: dict[int, float] = {}
+    cells_of = {s: d.nrows * d.ncols for s, d in doms.items()}
+    total_cells = sum(cells_of.values())
+    timeline = []                        # (wall s, members still running, their cells)
+    try:
+        batch.poll()
+        t0 = time.perf_counter()
+        while len(done) + len(failed) < len(doms):
+            _drive(batch, t_end, next_print, float(scens[0].print_step),
+                   lambda s, tim, steps, solver: done.__setitem__(s, steps), steps_per_poll,
+                   lambda s, tim: failed.__setitem__(s, tim))
+            running = [s for s in doms if s not in done and s not in failed]
+            timeline.append((time.perf_counter() - t0, len(running), sum(cells_of[s] for s in running)))
+        el = time.perf_counter() - t0
+        n, cells, upd = batch.info()
+    finally:
+        batch.close()
+        for d in doms.values():
+            d.close()
     out = {"scenarios": n, "cells": cells, "cell_updates": upd, "wall_s": el, "tiling": tiling,
-           "cell_updates_per_s": upd / el, "max_steps": max(done.values()),
-           "min_steps": min(done.values())}
-    batch.close()
-    for d in doms.values():
-        d.close()
+           "cell_updates_per_s": upd / el, "max_steps": max(done.values(), default=0),
+           "min_steps": min(done.values(), default=0), "failed_slots": sorted(failed),
+           "occupancy": occupancy_summary(timeline, len(doms), total_cells)}
     return out
+
+
+def occupancy_summary(timeline, members: int, total_cells: int) -> dict:
+    """How full a batch stayed over its run: the time-weighted mean fraction
+    of the members' cells still stepping, and the wall times at which the
+    running members fell to 50 % and 10 % of the batch."""
+    if not timeline:
+        return {}
+    t_prev, acc = 0.0, 0.0
+    t50 = t10 = None
+    for t, n, c in timeline:
+        acc += (t - t_prev) * c
+        t_prev = t
+        if t50 is None and n <= members * 0.5:
+            t50 = t
+        if t10 is None and n <= members * 0.1:
+            t10 = t
+    wall = timeline[-1][0]
+    return {"mean_active_cell_fraction": acc / (wall * total_cells) if wall > 0 else None,
+            "wall_s_at_half_members_done": t50, "wall_s_at_90pct_members_done": t10,
+            "samples": [[round(t, 3), n, c] for t, n, c in timeline[:: max(1, len(timeline) // 32)]]}
 
 
 def poisson_arrivals(rate: float, seconds: float, seed: int) -> np.ndarray:
@@ -146,9 +184,18 @@ def c4_stream(rate: float = 20.0, seconds: float = 300.0, seed: int = 4, pool: i
 
 
 def serve(requests: list[Request], budget_bytes: int = 200 * 10 ** 9, max_members: int = 1024,
-          steps_per_poll: int = 8, keep_results: bool = False) -> dict:
+          steps_per_poll: int = 8, keep_results: bool = False, prepare_hook=None) -> dict:
     """Continuous batching of a request stream (arrival times relative to the
-    start).  Returns latency percentiles and throughput."""
+    start).  Returns latency percentiles and throughput.
+
+    Admission is in arrival order while the device bytes of the live members
+    plus the next request fit ``budget_bytes``; a request that does not fit
+    waits (queues) until finished members free their memory.  A request that
+    alone needs more than the budget can never run: it is rejected at once.
+    A member whose state turns non-finite is halted by the device on its own
+    (FX_EV_ERROR), evicted and reported failed; the others keep running.
+    ``prepare_hook(req, dom)``, if given, is called on each request's domain
+    after its set-up and before admission."""
     from . import autotune
     pending = deque(sorted(requests, key=lambda r: r.arrival))
     batch = capi.Batch(nchem=1, ade=True, max_members=max_members)
@@ -158,53 +205,93 @@ def serve(requests: list[Request], budget_bytes: int = 200 * 10 ** 9, max_member
     used = 0
     peak_used = 0
     peak_live = 0
+    queued = 0                           # admissions that waited for memory or a slot
     finished: list[Request] = []
+    failed: list[Request] = []
     t0 = time.perf_counter()
 
     def now() -> float:
         return time.perf_counter() - t0
 
-    def on_done(slot, tim, steps, solver_steps):
+    def release(slot):
         nonlocal used
         req, dom, nb = live.pop(slot)
         batch.evict(slot)
+        used -= nb
+        return req, dom
+
+    def on_done(slot, tim, steps, solver_steps):
+        req, dom = release(slot)
         h = dom.get("h")                              # the result, copied back to the host
         if keep_results:
             req.result = h
         dom.close()
-        used -= nb
         req.steps = steps
         req.solver_steps = solver_steps
         req.finished = now()
         finished.append(req)
 
-    while pending or live:
-        # admission: arrived requests, in order, while memory and slots allow
-        while pending and pending[0].arrival <= now():
-            req = pending[0]
-            nb = domain_bytes(req.scenario)
-            if used + nb > budget_bytes or len(live) >= max_members:
-                break
-            pending.popleft()
-            # tile shape for how full the device will be with this member
-            live_cells = sum(r.cells for r, _, _ in live.values()) + req.cells
-            dom = capi.HipDomain.from_scenario(req.scenario, precision=capi.FP32_MIXED, ade=True,
-                                               init=req.init, tiling=autotune.choose_cells(live_cells))
-            slot = batch.admit(dom, float(req.scenario.t_end), float(req.scenario.print_step))
-            live[slot] = (req, dom, nb)
-            t_end[slot] = float(req.scenario.t_end)
-            next_print[slot] = float(req.scenario.print_step)
-            used += nb
-            req.admitted = now()
-        peak_used = max(peak_used, used)
-        peak_live = max(peak_live, len(live))
-        if not live:
-            if pending:
-                time.sleep(max(0.0, pending[0].arrival - now()))
-            continue
-        _drive(batch, t_end, next_print, float(requests[0].scenario.print_step), on_done, steps_per_poll)
-    wall = now()
-    batch.close()
+    def on_error(slot, tim):
+        req, dom = release(slot)
+        dom.close()
+        req.error = f"NaN/Inf in the state at t={tim:.3f}s"
+        req.finished = now()
+        failed.append(req)
+
+    try:
+        while pending or live:
+            # admission: arrived requests, in order, while memory and slots allow
+            while pending and pending[0].arrival <= now():
+                req = pending[0]
+                nb = domain_bytes(req.scenario)
+                if nb > budget_bytes:                 # can never fit: reject, do not wait
+                    pending.popleft()
+                    req.error = f"needs {nb} device bytes, above the {budget_bytes}-byte budget"
+                    req.finished = now()
+                    failed.append(req)
+                    continue
+                if used + nb > budget_bytes or len(live) >= max_members:
+                    if not req.queued:
+                        req.queued = True
+                        queued += 1
+                    break
+                pending.popleft()
+                # tile shape for how full the device will be with this member
+                live_cells = sum(r.cells for r, _, _ in live.values()) + req.cells
+                dom = capi.HipDomain.from_scenario(req.scenario, precision=capi.FP32_MIXED, ade=True,
+                                                   init=req.init, tiling=autotune.choose_cells(live_cells))
+                try:
+                    if prepare_hook is not None:
+                        prepare_hook(req, dom)
+                    slot = batch.admit(dom, float(req.scenario.t_end), float(req.scenario.print_step))
+                except Exception:
+                    dom.close()
+                    raise
+                live[slot] = (req, dom, nb)
+                t_end[slot] = float(req.scenario.t_end)
+                next_print[slot] = float(req.scenario.print_step)
+                used += nb
+                req.admitted = now()
+            peak_used = max(peak_used, used)
+            peak_live = max(peak_live, len(live))
+            if not live:
+                if pending:
+                    time.sleep(max(0.0, pending[0].arrival - now()))
+                continue
+            _drive(batch, t_end, next_print, float(requests[0].scenario.print_step), on_done, steps_per_poll,
+                   on_error)
+    finally:
+        wall = now()
+        doms = [dom for _, dom, _ in live.values()]
+        batch.close()                                 # members still in it leave with their state
+        for dom in doms:
+            dom.close()
+    out = {"requests": len(finished), "failed": len(failed), "queued_for_memory": queued, "wall_s": wall,
+           "mean_admission_wait_s": None, "cell_updates": 0, "cell_updates_per_s": 0.0,
+           "peak_members": peak_live, "peak_device_bytes": int(peak_used),
+           "failed_ids": [r.rid for r in failed], "errors": sorted({r.error for r in failed})}
+    if not finished:
+        return out
     lat = np.array([r.finished - r.arrival for r in finished])
     wait = np.array([r.admitted - r.arrival for r in finished])
     upd = sum(r.cells * r.solver_steps for r in finished)
@@ -213,10 +300,9 @@ def serve(requests: list[Request], budget_bytes: int = 200 * 10 ** 9, max_member
         ls = np.array([r.finished - r.arrival for r in finished if r.scenario.nrows == side])
         by_side[str(side)] = {"n": int(ls.size), "p50_s": float(np.percentile(ls, 50)),
                               "p99_s": float(np.percentile(ls, 99))}
-    return {"requests": len(finished), "wall_s": wall,
-            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
-            "p99_over_p50": float(np.percentile(lat, 99) / np.percentile(lat, 50)),
-            "mean_admission_wait_s": float(wait.mean()), "cell_updates": int(upd),
-            "cell_updates_per_s": upd / wall, "peak_members": peak_live,
-            "peak_device_bytes": int(peak_used), "by_side": by_side,
-            "mean_steps": float(np.mean([r.steps for r in finished]))}
+    out.update({"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+                "p99_over_p50": float(np.percentile(lat, 99) / np.percentile(lat, 50)),
+                "mean_admission_wait_s": float(wait.mean()), "max_admission_wait_s": float(wait.max()),
+                "cell_updates": int(upd), "cell_updates_per_s": upd / wall, "by_side": by_side,
+                "mean_steps": float(np.mean([r.steps for r in finished]))})
+    return out

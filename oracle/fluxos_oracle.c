@@ -52,6 +52,25 @@ int orc_num_threads(void)
 #endif
 }
 
+/* thread count of the hydro loops' parallel region (timing only) */
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#endif
+}
+
+/* the OpenMP thread-binding policy in effect (omp_proc_bind_t: 0 false,
+ * 1 true, 2 master/primary, 3 close, 4 spread) */
+int orc_proc_bind(void)
+{
+#ifdef _OPENMP
+    return (int)omp_get_proc_bind();
+#else
+    return 0;
+#endif
+}
+
 /* ------------------------------------------------------------------ */
 /* CFL condition: main.cpp:300-325                                      */
 /* ------------------------------------------------------------------ */
@@ -1002,15 +1021,31 @@ void orc_wintra(orc_domain *d, int ichem)
 /* ------------------------------------------------------------------ */
 /* one iteration of the time loop: main.cpp:271-406                     */
 /* ------------------------------------------------------------------ */
-int orc_step(orc_domain *d, double print_next, int32_t *it, double *hpall_out)
+static double orc_wtime(void)
+{
+#ifdef _OPENMP
+    return omp_get_wtime();
+#else
+    return 0.0;
+#endif
+}
+
+/* orc_step with the wall time of each phase added to phase_s[0..3] (CFL,
+ * add_meteo + add_inflow, hydrodynamics_calc, adesolver_calc + WINTRA) when
+ * phase_s is not NULL; the arithmetic is orc_step's */
+int orc_step_timed(orc_domain *d, double print_next, int32_t *it, double *hpall_out, double *phase_s)
 {
     int flags = 0;
+    double t0 = phase_s ? orc_wtime() : 0.0, t1;
     double hpall = orc_courant(d, print_next);
+    if (phase_s) { t1 = orc_wtime(); phase_s[0] += t1 - t0; t0 = t1; }
     orc_add_meteo(d);
     if (orc_add_inflow(d)) flags |= 2;
+    if (phase_s) { t1 = orc_wtime(); phase_s[1] += t1 - t0; t0 = t1; }
     if (hpall != 0) {
         (*it)++;
         orc_hydrodynamics(d);
+        if (phase_s) { t1 = orc_wtime(); phase_s[2] += t1 - t0; t0 = t1; }
         if (d->ade_solver) {
             double *scratch = (double *)calloc((size_t)d->mrows * d->mcols * 3, sizeof(double));
             for (int k = 0; k < d->nchem; k++) orc_adesolver(d, *it, k, scratch);
@@ -1020,8 +1055,14 @@ int orc_step(orc_domain *d, double print_next, int32_t *it, double *hpall_out)
          * solver (main.cpp:207-213) */
         if (d->wintra && d->ade_solver && d->soil)
             for (int k = 0; k < d->nchem; k++) orc_wintra(d, k);
+        if (phase_s) { t1 = orc_wtime(); phase_s[3] += t1 - t0; t0 = t1; }
         flags |= 1;
     }
     if (hpall_out) *hpall_out = hpall;
     return flags;
+}
+
+int orc_step(orc_domain *d, double print_next, int32_t *it, double *hpall_out)
+{
+    return orc_step_timed(d, print_next, it, hpall_out, NULL);
 }

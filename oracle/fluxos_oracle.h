@@ -162,6 +162,10 @@ void orc_wintra(orc_domain *d, int ichem);
 int orc_step(orc_domain *d, double print_next, int32_t *it, double *hpall_out);
 
 int orc_num_threads(void);
+/* timing helpers of the CPU baseline (not part of the restatement) */
+int orc_step_timed(orc_domain *d, double print_next, int32_t *it, double *hpall_out, double *phase_s);
+void orc_set_threads(int n);
+int orc_proc_bind(void);
 
 #ifdef __cplusplus
 }

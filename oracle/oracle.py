@@ -95,6 +95,10 @@ def load(path: Path | None = None) -> C.CDLL:
     lib.orc_wintra.restype = None
     lib.orc_step.argtypes = [P, C.c_double, C.POINTER(C.c_int32), _D]
     lib.orc_num_threads.restype = C.c_int
+    lib.orc_step_timed.argtypes = [P, C.c_double, C.POINTER(C.c_int32), _D, _D]
+    lib.orc_set_threads.argtypes = [C.c_int]
+    lib.orc_set_threads.restype = None
+    lib.orc_proc_bind.restype = C.c_int
     _LIBS[key] = lib
     return lib
 
@@ -235,6 +239,15 @@ class OracleDomain:
     def step(self, print_next: float) -> tuple[int, float]:
         hp = C.c_double(0.0)
         flags = self.lib.orc_step(C.byref(self.s), float(print_next), C.byref(self.it), C.byref(hp))
+        return flags, hp.value
+
+    def step_timed(self, print_next: float, phase_s: np.ndarray) -> tuple[int, float]:
+        """step() with the wall time of CFL, forcing, hydrodynamics and ADE
+        added to phase_s[0..3] (a float64 array of 4)."""
+        assert phase_s.dtype == np.float64 and phase_s.size == 4 and phase_s.flags.c_contiguous
+        hp = C.c_double(0.0)
+        flags = self.lib.orc_step_timed(C.byref(self.s), float(print_next), C.byref(self.it), C.byref(hp),
+                                        phase_s.ctypes.data_as(_D))
         return flags, hp.value
 
     @property
