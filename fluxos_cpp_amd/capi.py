@@ -44,7 +44,7 @@ EXPORTS = ("fx_domain_create", "fx_domain_destroy", "fx_domain_bytes", "fx_set_s
            "fx_batch_resume", "fx_batch_evict", "fx_batch_run", "fx_batch_poll", "fx_batch_info",
            "fx_read_dem", "fx_read_forcing", "fx_write_results", "fx_set_tiling", "fx_get_tiling",
            "fx_set_wintra", "fx_wintrasolver_calc", "fx_initiation_restart", "fx_read_results",
-           "fx_find_last_step", "fx_solute_balance")
+           "fx_find_last_step", "fx_solute_balance", "fx_mass_balance_noalloc")
 EV_PRINT, EV_DONE, EV_ERROR = 1, 2, 3
 K_CFL, K_MELT, K_FLUX, K_UPDATE, K_ADE_PASS1, K_ADE_SWEEP, K_STEP32, K_ADE32, K_CFL32 = range(9)
 KERNEL_NAMES = {K_CFL: "k64_cfl", K_MELT: "k64_melt", K_FLUX: "k64_flux", K_UPDATE: "k64_update",
@@ -130,6 +130,7 @@ def load(path: Path | None = None) -> C.CDLL:
         "fx_adesolver_calc": ([vp, C.c_int32, C.c_int32], C.c_int),
         "fx_advance": ([vp, C.c_double, C.c_double, C.c_int32, C.POINTER(StepInfo)], C.c_int),
         "fx_mass_balance": ([vp, _D], C.c_int),
+        "fx_mass_balance_noalloc": ([vp, _D], C.c_int),
         "fx_solute_balance": ([vp, C.c_int32, _D], C.c_int),
         "fx_set_tiling": ([vp, C.c_int32, C.c_int32], C.c_int),
         "fx_get_tiling": ([vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
@@ -420,6 +421,13 @@ class HipDomain:
     def mass_balance(self):
         out = (C.c_double * 3)()
         _check(self.lib.fx_mass_balance(self.h, out))
+        return out[0], out[1], out[2]
+
+    def mass_balance_noalloc(self):
+        """mass_balance() computed in the library's fixed device array, with no
+        allocation (bit-identical results)."""
+        out = (C.c_double * 3)()
+        _check(self.lib.fx_mass_balance_noalloc(self.h, out))
         return out[0], out[1], out[2]
 
     def solute_balance(self, k: int = 0):

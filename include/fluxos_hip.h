@@ -177,6 +177,17 @@ int fx_advance(fx_domain dom, double t_end, double print_next, int32_t max_steps
  * inflow cell's deficit filled up to the bed, common.cpp:324-325),
  * out[2]=net weir inflow.  The ledger is kept by FP32-mixed domains. */
 int fx_mass_balance(fx_domain dom, double out[3]);
+/* fx_mass_balance without any allocation (extension; no counterpart in the
+ * reference).  For callers that cannot allocate device or pinned memory: code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  It runs fx_mass_balance's volume kernel into a fixed device
+ * array of the library (2^20 block partials, 8 MiB per device, held until the
+ * process ends) and sums the partials on the device in the same order, so
+ * out[] is bit-identical to fx_mass_balance's.  A domain whose volume kernel
+ * has more blocks than that (FP32-mixed: ceil(NROWS/61) * ceil(NCOLS/ch);
+ * FP64: ceil(NROWS/64) * ceil(NCOLS/4)) is FX_ERR_ARG.  Calls on one device
+ * share the array: the library serializes them within a process.          */
+int fx_mass_balance_noalloc(fx_domain dom, double out[3]);
 /* Solute ichem's mass in the non-weir cells, sum of c h dxy^2 in FP64
  * (out[0]), and the running FP64 ledger of the solute the melt and the
  * inflow brought into water (out[1], FP32-mixed domains; 0 for FP64
