@@ -127,26 +127,28 @@ def test_domain_in_two_batches_refused():
     d.close()
 
 
-def _requests(n: int, t_end: int = 1200):
+def _requests(n: int, t_end: int = 1200, rate: float = 0.0):
     """n requests of one size (48^2), each its own DEM and melt rate, all
-    arriving at once."""
+    arriving at once, or as a seeded Poisson stream of the given rate [1/s]."""
+    gaps = np.random.default_rng(7).exponential(1.0 / rate, n) if rate > 0 else np.zeros(n)
     reqs = []
-    for i in range(n):
+    for i, t in enumerate(np.cumsum(gaps)):
         sc = _scen(48, 20 + i, 25.0 + 10.0 * i, t_end=t_end)
-        reqs.append(serve.Request(i, 0.0, sc, capi.prepare(sc)))
+        reqs.append(serve.Request(i, float(t), sc, capi.prepare(sc)))
     return reqs
 
 
 def test_serve_queues_on_budget_and_results_match_alone():
-    """A budget that holds two of the requests at a time: the others wait for
-    memory and then run; every result equals the request's run alone, whether
-    it ran beside another member or not."""
-    reqs = _requests(5)
+    """A Poisson stream (500 requests/s, so they arrive while the first ones
+    run) under a budget that holds two of the requests at a time: the others
+    wait for memory and then run; every result equals the request's run
+    alone, whether it ran beside another member or not."""
+    reqs = _requests(5, rate=500.0)
     nb = serve.domain_bytes(reqs[0].scenario)
     budget = nb * 5 // 2
     out = serve.serve(reqs, budget_bytes=budget, keep_results=True, steps_per_poll=4)
     assert out["requests"] == 5 and out["failed"] == 0, out
-    assert out["queued_for_memory"] == 3, out
+    assert out["queued_for_memory"] >= 2, out
     assert out["peak_members"] == 2, out
     assert out["peak_device_bytes"] <= budget
     assert max(r.admitted - r.arrival for r in reqs) > 0.0
