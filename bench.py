@@ -244,6 +244,58 @@ def accuracy(sc, init, snap, spans: str) -> dict:
     return {"from_t": round(snap["clock"][0], 2), "spans_s": out}
 
 
+def whole_run_accuracy(sc, init) -> list:
+    """FP32-mixed against FP64, both from t = 0, at every print time of the
+    whole run (no solutes: the hydrodynamics do not depend on them)."""
+    from fluxos_cpp_amd import capi
+    d64 = capi.HipDomain.from_scenario(sc, precision=capi.FP64_PARITY, ade=False, init=init)
+    d32 = capi.HipDomain.from_scenario(sc, precision=capi.FP32_MIXED, ade=False, init=init)
+    out = []
+    pn = float(sc.print_step)
+    while pn <= sc.t_end:
+        d64.advance(float(sc.t_end), pn)
+        d32.advance(float(sc.t_end), pn)
+        h64 = d64.get("h")[1:-1, 1:-1]
+        dh = np.abs(d32.get("h")[1:-1, 1:-1] - h64)
+        out.append({"t_s": pn, "fp32_max_abs_h_m": float(dh.max()), "fp32_rms_h_m": float(np.sqrt((dh ** 2).mean())),
+                    "max_h_m": float(h64.max()),
+                    "wet_fp64": int((d64.get("ldry")[1:-1, 1:-1] == 0).sum()),
+                    "wet_fp32": int((d32.get("ldry")[1:-1, 1:-1] == 0).sum())})
+        pn += sc.print_step
+    d64.close()
+    d32.close()
+    return out
+
+
+def accuracy_configs(spans: str) -> dict:
+    """FP32-mixed depth error against FP64 for each configuration's generator,
+    from a wet snapshot (FP32-mixed spin-up on the device) over the given
+    spans, beside FP64's one-ulp floor; C1 also over its whole run from t = 0."""
+    from fluxos_cpp_amd import capi, gen
+    cfgs = [("C1 256x256 (full size)", gen.scenario_c1(256), 3600.0),
+            ("C3 member 4 (256x256, 14.8 mm/day)", gen.scenario_c3(4), 1800.0),
+            ("C3 member 9 (288x288, 29.5 mm/day)", gen.scenario_c3(9), 1800.0),
+            ("C3 member 8 (362x362, 66.0 mm/day)", gen.scenario_c3(8), 1800.0),
+            ("C5 generator at 4096x4096 (dx 1 m, channels, melt multiplier)",
+             gen.scenario_c5(4096, t_end=10800), 1800.0)]
+    out = {}
+    for name, sc, t0 in cfgs:
+        t1 = time.perf_counter()
+        init = capi.prepare(sc)
+        d = capi.HipDomain.from_scenario(sc, precision=capi.FP32_MIXED, ade=False, init=init)
+        spin_up(d, t0, 10 ** 7)
+        snap = snapshot(d)
+        d.close()
+        row = accuracy(sc, init, snap, spans)
+        if name.startswith("C1"):
+            row["whole_run_from_t0"] = whole_run_accuracy(sc, init)
+        out[name] = row
+        log(f"[bench] accuracy {name}: {time.perf_counter() - t1:.1f}s; "
+            + ", ".join(f"{r['after_s']}s {r['fp32_max_abs_h_m']:.2e} (ulp {r['fp64_ulp_max_abs_h_m']:.2e})"
+                        for r in row["spans_s"]))
+    return out
+
+
 def largest_domain(nsteps: int, t_spin: float, spin_wall: float = 60.0) -> dict:
     """The largest single domain: 16384 x 16384, dx = 1 m, fractal DEM with
     carved channels, a smooth melt-multiplier field (10-60 mm/day), four
@@ -340,6 +392,8 @@ def main() -> int:
                     help="timed iterations of the 16384x16384 four-solute domain (0 = skip)")
     ap.add_argument("--c5-start-t", type=float, default=3600.0,
                     help="untimed spin-up of that domain [simulated s]")
+    ap.add_argument("--accuracy-configs", action="store_true",
+                    help="also measure FP32-mixed vs FP64 depth error for the C1, C3 and C5 generators")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (calibrates HBM counters)")
     args = ap.parse_args()
@@ -548,6 +602,10 @@ def main() -> int:
             serving["note"] = (f"reduced: {args.serve_seconds:g}-s arrival window of the stated 300 s "
                                "(the full stream is ~38 GPU-minutes at today's throughput)")
 
+    acc_cfg = None
+    if args.accuracy_configs:
+        acc_cfg = accuracy_configs(args.accuracy_spans)
+
     c5 = None
     if args.c5_steps > 0:
         c5 = largest_domain(args.c5_steps, args.c5_start_t)
@@ -566,7 +624,7 @@ def main() -> int:
         "kernels": kernels,
         "step_roofline_frac": round(step_frac, 4),
         "mass_balance": mass,
-        "fp64_parity": fp64, "fp32_vs_fp64": err,
+        "fp64_parity": fp64, "fp32_vs_fp64": err, "fp32_vs_fp64_by_config": acc_cfg,
         "cpu_baseline": cpu,
         "ensemble": ensemble,
         "serving": serving,

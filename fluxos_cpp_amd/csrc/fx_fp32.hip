@@ -47,48 +47,64 @@ namespace {
 struct consts32 {
     float hdry, g, nuem, ks, dx, idx;
     float half_dx_over_dt, arbase_over_dt;
+    double hdry64;
 };
 
-// solver_dry in FP32 (solver_drydomain.cpp:78-193); pressure-only or Ritter
-// inflow from a wet E / N neighbour.  dzbE = zb(E) - zb(P), dzbN likewise.
-__device__ inline void solver_dry32(const consts32 &K, float dp, float de, float dn, float dzbE,
-                                    float dzbN, bool wetE, bool wetN, float dxy_over_dt,
-                                    float *f)
+// A wet/dry front in FP64: the depths after melt of P, E and N and the bed
+// differences E-P and N-P, from which a front face forms its free-surface
+// difference (ze - zp = (de - dp) + dzb) and its mean depth over the face's
+// bed (ze - zbpe = de + dzb/2), as the reference does on z in FP64.  The
+// branch tests of the front faces (wet above hdry, inflow or pressure only)
+// and the Ritter depth then see what the FP64 path sees; the flux magnitudes
+// follow in FP32.
+struct front64 {
+    double dp, de, dn, dzbE, dzbN;
+};
+
+// Ritter inflow face state from a free-surface drop dz (solver_drydomain.cpp:
+// 127-138, solver_wetdomain.cpp:131-138): q = 0.296 dz sqrt(g dz) (8/27),
+// depth 0.444 dz (4/9); q^2/h is 0 where dz rounds to 0 in FP32
+__device__ inline float ritter_q(float g, float dz) { return 0.296f * dz * sqrtf(g * dz); }
+__device__ inline float q2_over_h(float q, float h) { return h > 0.f ? q * (q / h) : 0.f; }
+
+// solver_dry (solver_drydomain.cpp:78-193): pressure-only or Ritter inflow
+// from a wet E / N neighbour, decided in FP64 on the front
+__device__ inline void solver_dry32(const consts32 &K, const front64 &fr, bool wetE, bool wetN,
+                                    float dxy_over_dt, float *f)
 {
     const float g = K.g;
     float fe1 = 0.f, fe2 = 0.f, fn1 = 0.f, fn3 = 0.f;
     if (wetE) {
-        // hme = max(0, ze - zbpe); ze <= zp test on the free surface
-        float hme = fmaxf(0.f, de + 0.5f * dzbE);
-        if (hme > K.hdry) {
-            const float he = fmaxf(0.f, de);
-            const float dze = (de - dp) + dzbE;
-            if (dze <= 0.f) {
-                fe2 = .5f * g * hme * hme;
+        const double hme = fmax(0.0, fr.de + 0.5 * fr.dzbE);          // ze - zbpe
+        if (hme > K.hdry64) {
+            const double dze = (fr.de - fr.dp) + fr.dzbE;              // ze - zp
+            if (dze <= 0.0) {
+                const float hm = (float)hme;
+                fe2 = .5f * g * hm * hm;
             } else {
-                const float dz = fminf(fabsf(dze), hme);
-                float q = 0.296f * dz * sqrtf(g * dz);
-                hme = 0.444f * dz;
-                q = -fminf(q, dxy_over_dt * he);
+                const float dz = (float)fmin(dze, hme);
+                const float he = (float)fmax(0.0, fr.de);
+                const float hm = 0.444f * dz;
+                const float q = -fminf(ritter_q(g, dz), dxy_over_dt * he);
                 fe1 = q;
-                fe2 = q * (q / hme) + .5f * g * hme * hme;
+                fe2 = q2_over_h(q, hm) + .5f * g * hm * hm;
             }
         }
     }
     if (wetN) {
-        float hmn = fmaxf(0.f, dn + 0.5f * dzbN);
-        if (hmn > K.hdry) {
-            const float hn = fmaxf(0.f, dn);
-            const float dzn = (dn - dp) + dzbN;
-            if (dzn <= 0.f) {
-                fn3 = .5f * g * hmn * hmn;
+        const double hmn = fmax(0.0, fr.dn + 0.5 * fr.dzbN);
+        if (hmn > K.hdry64) {
+            const double dzn = (fr.dn - fr.dp) + fr.dzbN;
+            if (dzn <= 0.0) {
+                const float hm = (float)hmn;
+                fn3 = .5f * g * hm * hm;
             } else {
-                const float dz = fminf(fabsf(dzn), hmn);
-                float r = 0.296f * dz * sqrtf(g * dz);
-                hmn = 0.444f * dz;
-                r = -fminf(r, dxy_over_dt * hn);
+                const float dz = (float)fmin(dzn, hmn);
+                const float hn = (float)fmax(0.0, fr.dn);
+                const float hm = 0.444f * dz;
+                const float r = -fminf(ritter_q(g, dz), dxy_over_dt * hn);
                 fn1 = r;
-                fn3 = r * (r / hmn) + .5f * g * hmn * hmn;
+                fn3 = q2_over_h(r, hm) + .5f * g * hm * hm;
             }
         }
     }
@@ -110,7 +126,7 @@ struct nb32 {
     bool wetE, wetN;
 };
 
-__device__ inline void solver_wet32(const consts32 &K, const nb32 &n, float *f)
+__device__ inline void solver_wet32(const consts32 &K, const nb32 &n, const front64 &fr, float *f)
 {
     const float hdry = K.hdry, g = K.g, dx = K.dx;
     const float hp = fmaxf(0.f, n.dp);
@@ -125,35 +141,32 @@ __device__ inline void solver_wet32(const consts32 &K, const nb32 &n, float *f)
     float dzn = (n.dn - n.dp) + n.dzbN;
     const float drn = rn - rp, dqn = qn - qp;
     bool lroe = true;
+    // a dry E / N neighbour: the front face, decided in FP64 (:121-175)
     if (!n.wetE) {
-        hme = fmaxf(0.f, n.dp - 0.5f * n.dzbE);
+        const double hm64 = fmax(0.0, fr.dp - 0.5 * fr.dzbE);          // zp - zbpe
         he = 0.f; qe = 0.f; re = 0.f;
-        if (hme > hdry) {
-            if (dze >= 0.f) {
-                qme = 0.f; rme = 0.f;
-            } else {
-                dze = fminf(fabsf(dze), hme);
-                qme = fminf(0.296f * dze * sqrtf(g * dze), K.half_dx_over_dt * hp);
-                rme = 0.f;
+        qme = 0.f; rme = 0.f; hme = 0.f;
+        if (hm64 > K.hdry64) {
+            hme = (float)hm64;
+            const double dz64 = (fr.de - fr.dp) + fr.dzbE;               // ze - zp
+            if (dz64 < 0.0) {
+                const float dz = (float)fmin(-dz64, hm64);
+                qme = fminf(ritter_q(g, dz), K.half_dx_over_dt * hp);
             }
-        } else {
-            qme = 0.f; rme = 0.f; hme = 0.f;
         }
         lroe = false;
     }
     if (!n.wetN) {
-        hmn = fmaxf(0.f, n.dp - 0.5f * n.dzbN);
+        const double hm64 = fmax(0.0, fr.dp - 0.5 * fr.dzbN);
         hn = 0.f; qn = 0.f; rn = 0.f;
-        if (hmn > hdry) {
-            if (dzn >= 0.f) {
-                qmn = 0.f; rmn = 0.f;
-            } else {
-                dzn = fminf(fabsf(dzn), hmn);
-                rmn = fminf(0.296f * dzn * sqrtf(g * dzn), K.half_dx_over_dt * hp);
-                qmn = 0.f;
+        qmn = 0.f; rmn = 0.f; hmn = 0.f;
+        if (hm64 > K.hdry64) {
+            hmn = (float)hm64;
+            const double dz64 = (fr.dn - fr.dp) + fr.dzbN;
+            if (dz64 < 0.0) {
+                const float dz = (float)fmin(-dz64, hm64);
+                rmn = fminf(ritter_q(g, dz), K.half_dx_over_dt * hp);
             }
-        } else {
-            qmn = 0.f; rmn = 0.f; hmn = 0.f;
         }
         lroe = false;
     }
@@ -298,6 +311,20 @@ struct ring {
     float fn[2][3][NS];       // fn_1..fn_3 of columns c-1 and c
 };
 
+// the FP64 front of row i (columns of parity pc and pn = pc ^ 1): depths
+// after melt and bed differences, the bed as high + low parts in FP64
+__device__ inline front64 front_of(const ring &R, int pc, int pn, int i)
+{
+    front64 fr;
+    fr.dp = R.dm64[pc][i];
+    fr.de = R.dm64[pc][i + 1];
+    fr.dn = R.dm64[pn][i];
+    const double hp = (double)R.zbh[pc][i], lp = (double)R.zbl[pc][i];
+    fr.dzbE = ((double)R.zbh[pc][i + 1] - hp) + ((double)R.zbl[pc][i + 1] - lp);
+    fr.dzbN = ((double)R.zbh[pn][i] - hp) + ((double)R.zbl[pn][i] - lp);
+    return fr;
+}
+
 // flux m (0..5 = fe_1..fe_3, fn_1..fn_3) of row i of the column of parity par
 __device__ inline float &rflux(ring &R, int par, int m, int i)
 {
@@ -385,8 +412,6 @@ __device__ inline void flux_col(const sweep_ctx &X, ring &R, int c)
     if (X.gate && X.flux_lane && (fl & LI) && !(fl & LX)) {
         const bool wetE = R.fl[S0][i + 1] & LW, wetN = R.fl[SN][i] & LW;
         const int pc = c & 1, pn = pc ^ 1;     // slots of columns c and c+1
-        const float dzbE = (R.zbh[pc][i + 1] - R.zbh[pc][i]) + (R.zbl[pc][i + 1] - R.zbl[pc][i]);
-        const float dzbN = (R.zbh[pn][i] - R.zbh[pc][i]) + (R.zbl[pn][i] - R.zbl[pc][i]);
         if (!(fl & LW)) {
             const bool wetW = R.fl[S0][i - 1] & LW, wetS = R.fl[SS][i] & LW;
             if (!wetW && !wetE && !wetS && !wetN) {
@@ -395,10 +420,13 @@ __device__ inline void flux_col(const sweep_ctx &X, ring &R, int c)
                     for (int m = 0; m < 6; m++) f[m] = at_off(X.S->f6 + m * X.S->mm, p * 4u);
                 }
             } else {
-                solver_dry32(X.K, R.dm[pc][i], R.dm[pc][i + 1], R.dm[pn][i], dzbE, dzbN, wetE, wetN,
-                             X.dxy_over_dt, f);
+                solver_dry32(X.K, front_of(R, pc, pn, i), wetE, wetN, X.dxy_over_dt, f);
             }
         } else {
+            const float dzbE = (R.zbh[pc][i + 1] - R.zbh[pc][i]) + (R.zbl[pc][i + 1] - R.zbl[pc][i]);
+            const float dzbN = (R.zbh[pn][i] - R.zbh[pc][i]) + (R.zbl[pn][i] - R.zbl[pc][i]);
+            front64 fr;
+            if (!wetE || !wetN) fr = front_of(R, pc, pn, i);
             nb32 n;
             n.dp = R.dm[pc][i]; n.de = R.dm[pc][i + 1]; n.dn = R.dm[pn][i];
             n.dzbE = dzbE; n.dzbN = dzbN;
@@ -411,7 +439,7 @@ __device__ inline void flux_col(const sweep_ctx &X, ring &R, int c)
             n.vNW = R.v[pn][i - 1]; n.vNE = R.v[pn][i + 1];
             n.sqP = R.sq[pc][i]; n.sqE = R.sq[pc][i + 1]; n.sqN = R.sq[pn][i];
             n.wetE = wetE; n.wetN = wetN;
-            solver_wet32(X.K, n, f);
+            solver_wet32(X.K, n, fr, f);
         }
     }
     const int fs = c & 1;
@@ -565,6 +593,7 @@ __device__ inline void sweep_tile(const fx_params &P, const s32p &S, fx_ctl *ctl
     X.ld = (uint32_t)P.ld;
     X.hdryf = (float)P.hdry;
     X.K.hdry = (float)P.hdry;
+    X.K.hdry64 = P.hdry;
     X.K.g = (float)P.g;
     X.K.nuem = (float)P.nuem;
     X.K.ks = (float)P.ks0;
