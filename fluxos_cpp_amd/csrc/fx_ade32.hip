@@ -142,10 +142,12 @@ __device__ inline float a_face(float q, float cp, float cq, bool wq, float cu, b
 {
     if (!wq) return 0.f;                          // dry neighbour: fem = 0, no diffusion
     const float pd = -(sq * kdiff) * (cq - cp);
-    float fm;
-    if (q > 0.f) fm = wu ? -.125f * cu + .75f * cp + .375f * cq : .5f * (cp + cq);
-    else fm = wd ? .375f * cp + .75f * cq - .125f * cd : .5f * (cp + cq);
-    fm = fmaxf(0.f, fm);
+    // upwind-biased face value: -1/8 of the far upstream value, 3/4 of the
+    // upstream one and 3/8 of the downstream one, upstream being P for
+    // q > 0 and Q otherwise; the plain mean where the far value is dry
+    const bool pos = q > 0.f;
+    const float far = pos ? cu : cd, up = pos ? cp : cq, down = pos ? cq : cp;
+    const float fm = fx_max0((pos ? wu : wd) ? -.125f * far + .75f * up + .375f * down : .5f * (cp + cq));
     float pf = q * fm * dx + pd;
     if (pf < 0.f) pf = fmaxf(pf, cq * cap);
     return pf;
@@ -219,8 +221,8 @@ __device__ inline void a_column(const actx &X, awin<NC> &W, int c, bool upd, boo
     // floors, h*area/dt of the limiter and dt/(area*h) of the update
     float sE = 0.f, sN = 0.f, hA = 0.f, kup = 0.f;
     if (act) {
-        sE = sqrtf(fmaxf(gP.h * hE, 1e-4f));
-        sN = sqrtf(fmaxf(gP.h * gN.h, 1e-4f));
+        sE = fx_sqrt(fx_max_nn(gP.h * hE, 1e-4f));
+        sN = fx_sqrt(fx_max_nn(gP.h * gN.h, 1e-4f));
         hA = gP.h * X.area_dt;
         kup = X.dt_area * __builtin_amdgcn_rcpf(gP.h);
     }
@@ -256,7 +258,7 @@ __device__ inline void a_column(const actx &X, awin<NC> &W, int c, bool upd, boo
                 con = fmaxf(fminf(fminf(cW, crE), fminf(cs, cn)), con);
                 out = con * gP.h;
             }
-            bad |= X.transport && gP.w && !isfinite(out);   // reported as ctl->status bit 2
+            bad |= X.transport && gP.w && !fx_finitef(out);   // reported as ctl->status bit 2
             at_off(X.m_out[k], p * 4u) = out;
         }
         W.pn_s[k] = pn;

@@ -393,7 +393,7 @@ int fx_domain_destroy(fx_domain d)
     for (void *p : ptrs) dfree(p);
     for (int k = 0; k < FX_MAX_CHEM; k++) dfree(s.conc[k]);
     fx_state32 &t = d->s32;
-    void *ptrs32[] = {t.d[0], t.d[1], t.qx[0], t.qx[1], t.qy[0], t.qy[1], t.zb, t.zb_lo, t.f6, t.ff,
+    void *ptrs32[] = {t.d[0], t.d[1], t.q[0], t.q[1], t.zb2, t.f6, t.ff,
                       t.flags, t.tw_acc, t.tw_t0, t.partials, t.part2, t.ticket};
     for (void *p : ptrs32) dfree(p);
     for (int k = 0; k < FX_MAX_CHEM; k++) { dfree(t.mass[0][k]); dfree(t.mass[1][k]); }

@@ -64,8 +64,8 @@ struct front64 {
 // Ritter inflow face state from a free-surface drop dz (solver_drydomain.cpp:
 // 127-138, solver_wetdomain.cpp:131-138): q = 0.296 dz sqrt(g dz) (8/27),
 // depth 0.444 dz (4/9); q^2/h is 0 where dz rounds to 0 in FP32
-__device__ inline float ritter_q(float g, float dz) { return 0.296f * dz * sqrtf(g * dz); }
-__device__ inline float q2_over_h(float q, float h) { return h > 0.f ? q * (q / h) : 0.f; }
+__device__ inline float ritter_q(float g, float dz) { return 0.296f * dz * fx_sqrt(g * dz); }
+__device__ inline float q2_over_h(float q, float h) { return h > 0.f ? q * (q * fx_rcp(h)) : 0.f; }
 
 // solver_dry (solver_drydomain.cpp:78-193): pressure-only or Ritter inflow
 // from a wet E / N neighbour, decided in FP64 on the front
@@ -75,34 +75,34 @@ __device__ inline void solver_dry32(const consts32 &K, const front64 &fr, bool w
     const float g = K.g;
     float fe1 = 0.f, fe2 = 0.f, fn1 = 0.f, fn3 = 0.f;
     if (wetE) {
-        const double hme = fmax(0.0, fr.de + 0.5 * fr.dzbE);          // ze - zbpe
-        if (hme > K.hdry64) {
+        const double hme = fr.de + 0.5 * fr.dzbE;                      // ze - zbpe (its max with 0
+        if (hme > K.hdry64) {                                            // against hdry > 0)
             const double dze = (fr.de - fr.dp) + fr.dzbE;              // ze - zp
             if (dze <= 0.0) {
                 const float hm = (float)hme;
                 fe2 = .5f * g * hm * hm;
             } else {
-                const float dz = (float)fmin(dze, hme);
-                const float he = (float)fmax(0.0, fr.de);
+                const float dz = fx_min_nn((float)dze, (float)hme);
+                const float he = fx_max0((float)fr.de);
                 const float hm = 0.444f * dz;
-                const float q = -fminf(ritter_q(g, dz), dxy_over_dt * he);
+                const float q = -fx_min_nn(ritter_q(g, dz), dxy_over_dt * he);
                 fe1 = q;
                 fe2 = q2_over_h(q, hm) + .5f * g * hm * hm;
             }
         }
     }
     if (wetN) {
-        const double hmn = fmax(0.0, fr.dn + 0.5 * fr.dzbN);
+        const double hmn = fr.dn + 0.5 * fr.dzbN;
         if (hmn > K.hdry64) {
             const double dzn = (fr.dn - fr.dp) + fr.dzbN;
             if (dzn <= 0.0) {
                 const float hm = (float)hmn;
                 fn3 = .5f * g * hm * hm;
             } else {
-                const float dz = (float)fmin(dzn, hmn);
-                const float hn = (float)fmax(0.0, fr.dn);
+                const float dz = fx_min_nn((float)dzn, (float)hmn);
+                const float hn = fx_max0((float)fr.dn);
                 const float hm = 0.444f * dz;
-                const float r = -fminf(ritter_q(g, dz), dxy_over_dt * hn);
+                const float r = -fx_min_nn(ritter_q(g, dz), dxy_over_dt * hn);
                 fn1 = r;
                 fn3 = q2_over_h(r, hm) + .5f * g * hm * hm;
             }
@@ -129,9 +129,9 @@ struct nb32 {
 __device__ inline void solver_wet32(const consts32 &K, const nb32 &n, const front64 &fr, float *f)
 {
     const float hdry = K.hdry, g = K.g, dx = K.dx;
-    const float hp = fmaxf(0.f, n.dp);
-    float he = fmaxf(0.f, n.de);
-    float hn = fmaxf(0.f, n.dn);
+    const float hp = fx_max0(n.dp);
+    float he = fx_max0(n.de);
+    float hn = fx_max0(n.dn);
     float qp = n.qp, qe = n.qe, qn = n.qn;
     float rp = n.rp, re = n.re, rn = n.rn;
     float hme = .5f * (hp + he), qme = .5f * (qp + qe), rme = .5f * (rp + re);
@@ -143,35 +143,35 @@ __device__ inline void solver_wet32(const consts32 &K, const nb32 &n, const fron
     bool lroe = true;
     // a dry E / N neighbour: the front face, decided in FP64 (:121-175)
     if (!n.wetE) {
-        const double hm64 = fmax(0.0, fr.dp - 0.5 * fr.dzbE);          // zp - zbpe
+        const double hm64 = fr.dp - 0.5 * fr.dzbE;                     // zp - zbpe
         he = 0.f; qe = 0.f; re = 0.f;
         qme = 0.f; rme = 0.f; hme = 0.f;
         if (hm64 > K.hdry64) {
             hme = (float)hm64;
             const double dz64 = (fr.de - fr.dp) + fr.dzbE;               // ze - zp
             if (dz64 < 0.0) {
-                const float dz = (float)fmin(-dz64, hm64);
-                qme = fminf(ritter_q(g, dz), K.half_dx_over_dt * hp);
+                const float dz = fx_min_nn((float)(-dz64), hme);
+                qme = fx_min_nn(ritter_q(g, dz), K.half_dx_over_dt * hp);
             }
         }
         lroe = false;
     }
     if (!n.wetN) {
-        const double hm64 = fmax(0.0, fr.dp - 0.5 * fr.dzbN);
+        const double hm64 = fr.dp - 0.5 * fr.dzbN;
         hn = 0.f; qn = 0.f; rn = 0.f;
         qmn = 0.f; rmn = 0.f; hmn = 0.f;
         if (hm64 > K.hdry64) {
             hmn = (float)hm64;
             const double dz64 = (fr.dn - fr.dp) + fr.dzbN;
             if (dz64 < 0.0) {
-                const float dz = (float)fmin(-dz64, hm64);
-                rmn = fminf(ritter_q(g, dz), K.half_dx_over_dt * hp);
+                const float dz = fx_min_nn((float)(-dz64), hmn);
+                rmn = fx_min_nn(ritter_q(g, dz), K.half_dx_over_dt * hp);
             }
         }
         lroe = false;
     }
-    const float cme = sqrtf(g * hme), cmn = sqrtf(g * hmn);
-    const float ihme = 1.f / fmaxf(hme, hdry), ihmn = 1.f / fmaxf(hmn, hdry);
+    const float cme = fx_sqrt(g * hme), cmn = fx_sqrt(g * hmn);
+    const float ihme = fx_rcp(fx_max_nn(hme, hdry)), ihmn = fx_rcp(fx_max_nn(hmn, hdry));
     const float ume = qme * ihme, vme = rme * ihme, umn = qmn * ihmn, vmn = rmn * ihmn;
     // eddy coefficients: us is identically zero, so cvdef*us*h + nuem = nuem;
     // a dry neighbour contributes h = 0 and velocity 0 (:124-126,151-153)
@@ -188,8 +188,8 @@ __device__ inline void solver_wet32(const consts32 &K, const nb32 &n, const fron
         float cc = 0.f;
         if (hme > hdry) {
             float dzea = fabsf(dze);
-            if (dzea > 0.5f * hme) dze = copysignf(fminf(dzea, fabsf(he - hp)), dze);
-            cc = .25f / cme;
+            if (dzea > 0.5f * hme) dze = copysignf(fx_min_nn(dzea, fabsf(he - hp)), dze);
+            cc = .25f * fx_rcp(cme);
         }
         {
             const float c1 = ume, c2 = ume + cme, c3 = ume - cme;
@@ -204,8 +204,8 @@ __device__ inline void solver_wet32(const consts32 &K, const nb32 &n, const fron
         }
         cc = 0.f;
         if (hmn > hdry) {
-            dzn = copysignf(fminf(fabsf(dzn), fabsf(hn - hp)), dzn);
-            cc = .25f / cmn;
+            dzn = copysignf(fx_min_nn(fabsf(dzn), fabsf(hn - hp)), dzn);
+            cc = .25f * fx_rcp(cmn);
         }
         {
             const float c1 = vmn, c2 = vmn + cmn, c3 = vmn - cmn;
@@ -227,14 +227,14 @@ __device__ inline void solver_wet32(const consts32 &K, const nb32 &n, const fron
         if (fe1 > 0.f && fn1 > 0.f) {
             const float tot = (fe1 + fn1) * dx;
             if (tot > volrat) {
-                const float cf = fn1 / (fe1 + fn1);
+                const float cf = fn1 * fx_rcp(fe1 + fn1);
                 fe1 = (1.f - cf) * volrat * idx;
                 fn1 = cf * volrat * idx;
             }
         } else if (fe1 > 0.f) {
-            fe1 = fminf(fe1 * dx, volrat) * idx;
+            fe1 = fx_min_nn(fe1 * dx, volrat) * idx;
         } else if (fn1 > 0.f) {
-            fn1 = fminf(fn1 * dx, volrat) * idx;
+            fn1 = fx_min_nn(fn1 * dx, volrat) * idx;
         }
     } else {
         fe1 = 0.f; fn1 = 0.f;
@@ -258,7 +258,8 @@ __device__ inline double wsum(double v)
     return v;
 }
 
-// CFL bound of one updated cell (main.cpp:308-316), FP64
+// CFL bound of one updated cell (main.cpp:308-316), FP64; h is the depth or
+// the signed d = z - zb (the same above hdry > 0)
 __device__ inline void cfl_cell(const fx_params &P, double h, float qx, float qy, double &dtm, double &hmx)
 {
     if (h > P.hdry) {
@@ -266,9 +267,9 @@ __device__ inline void cfl_cell(const fx_params &P, double h, float qx, float qy
         // min(a/(u0+c0), a/(v0+c0)) = a/(max(u0,v0)+c0) (division is monotonic);
         // the bound is evaluated in FP32, the reduction over cells in FP64
         const float hf = (float)h;
-        const float c0 = sqrtf((float)P.g * hf);
-        const float u = fmaxf(1e-6f, fmaxf(fabsf(qx), fabsf(qy)) / hf);
-        dtm = fmin(dtm, (double)(P.cfl_dxy / (u + c0)));
+        const float c0 = fx_sqrt((float)P.g * hf);
+        const float u = fx_max_nn(1e-6f, fx_max_nn(fabsf(qx), fabsf(qy)) * fx_rcp(hf));
+        dtm = fmin(dtm, (double)(P.cfl_dxy * fx_rcp(u + c0)));
     }
 }
 
@@ -337,8 +338,8 @@ struct sweep_ctx {
     const s32p *S;
     const double *d_in;
     double *d_out;
-    const float *qx_in, *qy_in;
-    float *qx_out, *qy_out;
+    const float2 *q_in;
+    float2 *q_out;
     consts32 K;
     double meteoi, inflowi, s_wet_prev;
     uint32_t inflow_p;        // linear index of the inflow cell, ~0u = none
@@ -357,10 +358,12 @@ __device__ inline void load_col(const sweep_ctx &X, int c, pref &a)
     const s32p &S = *X.S;
     a.f = at_off(S.flags, p);
     a.d = at_off(X.d_in, p * 8u);
-    a.qx = at_off(X.qx_in, o4);
-    a.qy = at_off(X.qy_in, o4);
-    a.zbh = at_off(S.zb, o4);
-    a.zbl = at_off(S.zb_lo, o4);
+    const float2 q = at_off(X.q_in, p * 8u);
+    const float2 z = at_off(S.zb2, p * 8u);
+    a.qx = q.x;
+    a.qy = q.y;
+    a.zbh = z.x;
+    a.zbl = z.y;
     a.mult = X.P->melt_mult ? at_off(X.P->melt_mult, o4) : 1.0f;
     if (p == X.inflow_p) a.f |= PF_INFLOW;
 }
@@ -386,14 +389,14 @@ __device__ inline void store_col(const sweep_ctx &X, ring &R, const pref &a, int
     float qx = inner ? a.qx : 0.f, qy = inner ? a.qy : 0.f;
     if (X.gate && !wet) { qx = 0.f; qy = 0.f; }
     const float dm = (float)d;
-    const float h = fmaxf(0.f, dm);
-    const float den = __builtin_amdgcn_rcpf(fmaxf(fmaxf(h, X.hdryf), X.K.ks));
+    const float h = fx_max0(dm);
+    const float den = fx_rcp(fx_max_nn(fx_max_nn(h, X.hdryf), X.K.ks));
     R.dm[par][i] = dm;
     R.qx[par][i] = qx;
     R.qy[par][i] = qy;
     R.u[SL][i] = qx * den;
     R.v[par][i] = qy * den;
-    R.sq[par][i] = sqrtf(h);
+    R.sq[par][i] = fx_sqrt(h);
     R.zbh[par][i] = inner ? a.zbh : 0.f;
     R.zbl[par][i] = inner ? a.zbl : 0.f;
     R.fl[SL][i] = (uint8_t)((wet ? LW : 0) | ((a.f & FXF_WEIR) ? LX : 0) | (inner ? LI : 0) |
@@ -490,10 +493,11 @@ __device__ inline void update_col(const sweep_ctx &X, ring &R, int c, double &dt
             if (R.fl[S0][i + 1] & LX) wv -= (double)b1;        // into / out of a weir cell
             if (R.fl[SN][i] & LX) wv -= (double)e1;
         }
-        const double hd = fmax(0.0, d);
+        // h = max(0, d) against hdry > 0 is d against hdry
         // the reference's loop-4 wet flag (hydrodynamics_calc.cpp:123-154)
-        nf = (hd < P.hdry || weir) ? (uint8_t)(nf | FXF_DRY4) : (uint8_t)(nf & ~FXF_DRY4);
-        if (hd < P.hdry || weir) {
+        const bool dry4 = d < P.hdry || weir;
+        nf = dry4 ? (uint8_t)(nf | FXF_DRY4) : (uint8_t)(nf & ~FXF_DRY4);
+        if (dry4) {
             qxn = 0.f;
             qyn = 0.f;
         } else {
@@ -502,7 +506,7 @@ __device__ inline void update_col(const sweep_ctx &X, ring &R, int c, double &dt
         }
         // keep the fluxes of a cell that may be all-dry next step (S1)
         const bool wet1 = fl & LW;
-        if (!weir && (hd <= P.hdry || !P.rates_nonneg)) {
+        if (!weir && (d <= P.hdry || !P.rates_nonneg)) {
             const bool alldry = !wet1 && !(R.fl[S0][i - 1] & LW) && !(R.fl[S0][i + 1] & LW) &&
                                 !(R.fl[SS][i] & LW) && !(R.fl[SN][i] & LW);
             if (!alldry) {
@@ -532,16 +536,15 @@ __device__ inline void update_col(const sweep_ctx &X, ring &R, int c, double &dt
     // are left out: they never compute, and initiation leaves the bed of the
     // far corner NaN (0/0 of four NODATA vertices, initiate.cpp:85), so its
     // water surface is NaN in the reference too
-    bad |= !weir && (!isfinite(d) || !isfinite(qxn) || !isfinite(qyn));
+    bad |= !weir && (!fx_finite(d) || !fx_finitef(qxn) || !fx_finitef(qyn));
     at_off(X.d_out, p * 8u) = d;
-    at_off(X.qx_out, o4) = qxn;
-    at_off(X.qy_out, o4) = qyn;
+    at_off(X.q_out, p * 8u) = make_float2(qxn, qyn);
     if (X.ade_on) {
         at_off(S.ff, o4) = fe1P;
         at_off(S.ff + S.mm, o4) = fn1P;
     }
     if (nf != f) at_off(S.flags, p) = nf;
-    cfl_cell(P, fmax(0.0, d), qxn, qyn, dtm, hmx);
+    cfl_cell(P, d, qxn, qyn, dtm, hmx);
 }
 
 // one column of the sweep; SC = ring slot of column c
@@ -571,10 +574,8 @@ __device__ inline void sweep_tile(const fx_params &P, const s32p &S, fx_ctl *ctl
     X.S = &S;
     X.d_in = S.d_in[cur];
     X.d_out = const_cast<double *>(S.d_in[cur ^ 1]);
-    X.qx_in = S.qx_in[cur];
-    X.qy_in = S.qy_in[cur];
-    X.qx_out = const_cast<float *>(S.qx_in[cur ^ 1]);
-    X.qy_out = const_cast<float *>(S.qy_in[cur ^ 1]);
+    X.q_in = S.q_in[cur];
+    X.q_out = const_cast<float2 *>(S.q_in[cur ^ 1]);
     X.gate = ctl->gate;
     X.ade_on = ade_on;
     X.meteoi = P.meteo ? ctl->meteoi : 0.0;
@@ -662,7 +663,7 @@ __global__ __launch_bounds__(64) void k32_cfl(fx_params P, s32p S, fx_ctl *ctl)
 {
     const int cur = ctl->cur;
     const double *d_in = S.d_in[cur];
-    const float *qx_in = S.qx_in[cur], *qy_in = S.qy_in[cur];
+    const float2 *q_in = S.q_in[cur];
     const int k = threadIdx.x;
     const int r0 = 1 + blockIdx.x * TH, c0 = 1 + blockIdx.y * P.ch;
     const int row = r0 - 2 + k;
@@ -670,7 +671,7 @@ __global__ __launch_bounds__(64) void k32_cfl(fx_params P, s32p S, fx_ctl *ctl)
     if (k >= 2 && k <= NS - 2 && row <= P.nrows) {
         for (int c = c0; c <= min(c0 + P.ch - 1, P.ncols); c++) {
             const uint32_t p = (uint32_t)row + (uint32_t)c * (uint32_t)P.ld;
-            cfl_cell(P, fmax(0.0, d_in[p]), qx_in[p], qy_in[p], dtm, hmx);
+            cfl_cell(P, d_in[p], q_in[p].x, q_in[p].y, dtm, hmx);
         }
     }
     dtm = wmin(dtm);
@@ -954,10 +955,8 @@ s32p make_s32p(fx_domain d)
     fx_state32 &t = d->s32;
     s32p S;
     S.d_in[0] = t.d[0]; S.d_in[1] = t.d[1];
-    S.qx_in[0] = t.qx[0]; S.qx_in[1] = t.qx[1];
-    S.qy_in[0] = t.qy[0]; S.qy_in[1] = t.qy[1];
-    S.zb = t.zb;
-    S.zb_lo = t.zb_lo;
+    S.q_in[0] = t.q[0]; S.q_in[1] = t.q[1];
+    S.zb2 = t.zb2;
     S.f6 = t.f6;
     S.ff = t.ff;
     S.flags = t.flags;
@@ -1024,11 +1023,9 @@ int fx32_init_state(fx_domain d)
     int rc;
     for (int k = 0; k < 2; k++) {
         if ((rc = dalloc32(d, &t.d[k], mm))) return rc;
-        if ((rc = dalloc32(d, &t.qx[k], mm))) return rc;
-        if ((rc = dalloc32(d, &t.qy[k], mm))) return rc;
+        if ((rc = dalloc32(d, &t.q[k], mm))) return rc;
     }
-    if ((rc = dalloc32(d, &t.zb, mm))) return rc;
-    if ((rc = dalloc32(d, &t.zb_lo, mm))) return rc;
+    if ((rc = dalloc32(d, &t.zb2, mm))) return rc;
     if ((rc = dalloc32(d, &t.f6, 6 * mm))) return rc;
     if ((rc = dalloc32(d, &t.ff, 2 * mm))) return rc;
     if ((rc = dalloc32(d, &t.flags, mm))) return rc;
@@ -1058,14 +1055,12 @@ int fx32_set_field(fx_domain d, int field, const void *host)
     if ((rc = get_cur(d, &cur))) return rc;
     if (field == FX_F_ZB) {          // from fx_set_static: FP32 bed, flags base
         const double *zb = (const double *)host;
-        std::vector<float> zf(mm);
-        std::vector<float> zl(mm);
+        std::vector<float2> z2(mm);
         for (int64_t i = 0; i < mm; i++) {
-            zf[i] = (float)zb[i];
-            zl[i] = (float)(zb[i] - (double)zf[i]);
+            z2[i].x = (float)zb[i];
+            z2[i].y = (float)(zb[i] - (double)z2[i].x);
         }
-        FX_HIP(hipMemcpy(t.zb, zf.data(), mm * 4, hipMemcpyHostToDevice));
-        FX_HIP(hipMemcpy(t.zb_lo, zl.data(), mm * 4, hipMemcpyHostToDevice));
+        FX_HIP(hipMemcpy(t.zb2, z2.data(), mm * 8, hipMemcpyHostToDevice));
         return FX_OK;
     }
     if (field == -1) {               // weir mask (uint8 0/1) -> flag bytes
@@ -1098,10 +1093,13 @@ int fx32_set_field(fx_domain d, int field, const void *host)
     case FX_F_QX:
     case FX_F_QY: {
         const double *q = (const double *)host;
-        std::vector<float> qf(mm);
-        for (int64_t i = 0; i < mm; i++) qf[i] = (float)q[i];
-        float *dst = field == FX_F_QX ? t.qx[cur] : t.qy[cur];
-        FX_HIP(hipMemcpy(dst, qf.data(), mm * 4, hipMemcpyHostToDevice));
+        std::vector<float2> q2(mm);
+        FX_HIP(hipMemcpy(q2.data(), t.q[cur], mm * 8, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < mm; i++) {
+            if (field == FX_F_QX) q2[i].x = (float)q[i];
+            else q2[i].y = (float)q[i];
+        }
+        FX_HIP(hipMemcpy(t.q[cur], q2.data(), mm * 8, hipMemcpyHostToDevice));
         return FX_OK;
     }
     case FX_F_FE1: case FX_F_FE2: case FX_F_FE3: case FX_F_FN1: case FX_F_FN2: case FX_F_FN3: {
@@ -1191,9 +1189,9 @@ int fx32_get_field(fx_domain d, int field, void *host)
     }
     case FX_F_QX:
     case FX_F_QY: {
-        std::vector<float> q(mm);
-        FX_HIP(hipMemcpy(q.data(), field == FX_F_QX ? t.qx[cur] : t.qy[cur], mm * 4, hipMemcpyDeviceToHost));
-        for (int64_t i = 0; i < mm; i++) out[i] = q[i];
+        std::vector<float2> q2(mm);
+        FX_HIP(hipMemcpy(q2.data(), t.q[cur], mm * 8, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < mm; i++) out[i] = field == FX_F_QX ? q2[i].x : q2[i].y;
         return FX_OK;
     }
     case FX_F_LDRY: {                // as the last solver step left it

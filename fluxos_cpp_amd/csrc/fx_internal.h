@@ -70,9 +70,9 @@ struct fx_state64 {
 // and fluxes in FP32, bed in FP32, per-cell flag byte.
 struct fx_state32 {
     double *d[2];              // z - zb, signed (the reference keeps deficits)
-    float *qx[2], *qy[2];
-    float *zb;                 // bed elevation, FP32 high part
-    float *zb_lo;              // zb - (double)zb_hi: bed differences exact to ~1e-12 m
+    float2 *q[2];              // (qx, qy) of each cell, one 8-byte load
+    float2 *zb2;               // bed elevation: FP32 high part, and the low part
+                               // zb - (double)high, so bed differences are exact to ~1e-12 m
     float *f6;                 // persistent fluxes fe1,fe2,fe3,fn1,fn2,fn3 (S1), SoA planes
     float *ff;                 // fe1 / fn1 of the step for the fast ADE (two planes)
     float *mass[2][FX_MAX_CHEM];  // solute mass per unit area (c*h), ping-pong

@@ -7,9 +7,8 @@
 // pointers of one domain's FP32-mixed state as the step sweep sees them
 struct s32p {
     const double *d_in[2];
-    const float *qx_in[2], *qy_in[2];
-    const float *zb;
-    const float *zb_lo;
+    const float2 *q_in[2];          // (qx, qy)
+    const float2 *zb2;              // bed high and low parts
     float *f6;
     float *ff;
     uint8_t *flags;
@@ -45,6 +44,29 @@ __device__ inline const T &at_off(const T *base, uint32_t off)
 {
     return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
 }
+
+// finiteness by the exponent bits, which no floating-point option of the
+// compiler can fold away (the FP32 kernels are built without NaN semantics)
+__device__ inline bool fx_finite(double v)
+{
+    return (((unsigned long long)__double_as_longlong(v) >> 52) & 0x7ffull) != 0x7ffull;
+}
+__device__ inline bool fx_finitef(float v)
+{
+    return (((unsigned)__float_as_uint(v) >> 23) & 0xffu) != 0xffu;
+}
+
+// One-instruction forms for the FP32 kernels.  max(0, x), and min / max of
+// two floats of which at least one is non-negative (>= +0), compare the bit
+// patterns as signed integers: for such pairs the integer order is the float
+// order, and no NaN quieting of the operands is needed as for fmaxf / fminf.
+// The reciprocal and square root are the hardware's (1 ulp; FP32 denormals
+// are flushed in these kernels anyway).
+__device__ inline float fx_max0(float x) { return __int_as_float(max(0, __float_as_int(x))); }
+__device__ inline float fx_max_nn(float a, float b) { return __int_as_float(max(__float_as_int(a), __float_as_int(b))); }
+__device__ inline float fx_min_nn(float a, float b) { return __int_as_float(min(__float_as_int(a), __float_as_int(b))); }
+__device__ inline float fx_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline float fx_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 #define FX32_TH 61            // rows owned per step-sweep strip
 #define FX32_CH 32            // default columns per step-sweep block
