@@ -84,8 +84,9 @@ __device__ inline void a_load(const actx &X, int c, apref<NC> &a)
     a.f = at_off(A.flags, p);
     a.dold = at_off(X.d_old, p * 8u);
     a.dnew = at_off(X.d_new, p * 8u);
-    a.fe = at_off(A.ff, o4);
-    a.fn = at_off(A.ff + A.mm, o4);
+    const float2 ff = at_off(A.ff, p * 8u);
+    a.fe = ff.x;
+    a.fn = ff.y;
     a.mult = X.P->melt_mult ? at_off(X.P->melt_mult, o4) : 1.0f;
     if (p == X.inflow_p) a.f |= AF_INFLOW;
 #pragma unroll

@@ -147,7 +147,7 @@ int64_t fp64_bytes(const fx_domain_desc &D)
 int64_t fp32_bytes(const fx_domain_desc &D)
 {
     const int64_t mm = (int64_t)(D.nrows + 2) * (D.ncols + 2);
-    // d x2 (f64), qx,qy x2, zb hi+lo, f6 (6 planes), ff (2 planes), tw x2, flags
+    // d x2 (f64), (qx,qy) x2, zb (f64), f6 (6 per cell), ff (2 planes), tw x2, flags
     int64_t b = mm * 16 + mm * 4 * 4 + mm * 8 + mm * 4 * 6 + mm * 4 * 2 + mm * 4 * 2 + mm;
     if (D.ade_solver) b += mm * 8 * std::max(1, D.nchem);
     return b;
@@ -393,8 +393,8 @@ int fx_domain_destroy(fx_domain d)
     for (void *p : ptrs) dfree(p);
     for (int k = 0; k < FX_MAX_CHEM; k++) dfree(s.conc[k]);
     fx_state32 &t = d->s32;
-    void *ptrs32[] = {t.d[0], t.d[1], t.q[0], t.q[1], t.zb2, t.f6, t.ff,
-                      t.flags, t.tw_acc, t.tw_t0, t.partials, t.part2, t.ticket};
+    void *ptrs32[] = {t.d[0], t.d[1], t.q[0], t.q[1], t.zb, t.f6, t.ff,
+                      t.flags, t.tw, t.partials, t.part2, t.ticket};
     for (void *p : ptrs32) dfree(p);
     for (int k = 0; k < FX_MAX_CHEM; k++) { dfree(t.mass[0][k]); dfree(t.mass[1][k]); }
     for (auto &r : d->prof.pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }

@@ -2,7 +2,7 @@
 // kernel per time step.
 //
 // State in HBM (per cell): depth relative to the bed d = z - zb in FP64
-// (ping-pong), qx, qy in FP32 (ping-pong), bed zb in FP32, one flag byte, and
+// (ping-pong), qx, qy in FP32 (ping-pong), bed zb in FP64, one flag byte, and
 // -- touched only where needed -- the persistent fluxes of cells that may be
 // "all dry" next step (the reference keeps stale fluxes there,
 // solver_drydomain.cpp:60-76) and the wet-time tracer (written only when a
@@ -20,11 +20,10 @@
 // Face volume terms are FP32 products summed exactly in FP64, so every face
 // moves the same volume out of one cell and into the other and the FP64
 // depth conserves water.
-// The bed is held as an FP32 high part plus an FP32 low part: differences of
-// neighbouring high parts are exact (Sterbenz) and the low parts restore the
-// FP64 bed to ~1e-12 m, so the water-surface gradient of millimetre films over
-// a 500 m datum keeps its accuracy and the wet/dry and Ritter thresholds of
-// the face fluxes see the bed as the FP64 path does.
+// The bed is held in FP64 and only its differences between neighbours enter
+// the FP32 arithmetic, so the water-surface gradient of millimetre films over
+// a 500 m datum keeps its accuracy (~1e-13 m), and the front faces, which
+// decide in FP64, see the bed as the FP64 path does.
 #include "fx_kern32.h"
 
 #include <cstdlib>
@@ -291,7 +290,8 @@ __device__ inline void cfl_cell(const fx_params &P, double h, float qx, float qy
 
 struct pref {                 // one cell's HBM values in flight
     double d;
-    float qx, qy, zbh, zbl, mult;
+    double zb;
+    float qx, qy, mult;
     uint32_t f;               // flag byte, plus PF_INFLOW
 };
 #define PF_INFLOW 0x100u      // the cell takes the inflow point source
@@ -304,7 +304,7 @@ struct pref {                 // one cell's HBM values in flight
 struct ring {
     float u[3][NS];
     float dm[2][NS], qx[2][NS], qy[2][NS], v[2][NS], sq[2][NS];
-    float zbh[2][NS], zbl[2][NS];
+    double zb[2][NS];          // FP64 bed
     uint8_t fl[3][NS];        // LW | LX | LI | LS
     uint8_t raw[2][NS];       // flag bytes
     double dm64[2][NS];       // FP64 depth after melt
@@ -313,16 +313,15 @@ struct ring {
 };
 
 // the FP64 front of row i (columns of parity pc and pn = pc ^ 1): depths
-// after melt and bed differences, the bed as high + low parts in FP64
+// after melt and bed differences
 __device__ inline front64 front_of(const ring &R, int pc, int pn, int i)
 {
     front64 fr;
     fr.dp = R.dm64[pc][i];
     fr.de = R.dm64[pc][i + 1];
     fr.dn = R.dm64[pn][i];
-    const double hp = (double)R.zbh[pc][i], lp = (double)R.zbl[pc][i];
-    fr.dzbE = ((double)R.zbh[pc][i + 1] - hp) + ((double)R.zbl[pc][i + 1] - lp);
-    fr.dzbN = ((double)R.zbh[pn][i] - hp) + ((double)R.zbl[pn][i] - lp);
+    fr.dzbE = R.zb[pc][i + 1] - R.zb[pc][i];
+    fr.dzbN = R.zb[pn][i] - R.zb[pc][i];
     return fr;
 }
 
@@ -359,11 +358,9 @@ __device__ inline void load_col(const sweep_ctx &X, int c, pref &a)
     a.f = at_off(S.flags, p);
     a.d = at_off(X.d_in, p * 8u);
     const float2 q = at_off(X.q_in, p * 8u);
-    const float2 z = at_off(S.zb2, p * 8u);
+    a.zb = at_off(S.zb, p * 8u);
     a.qx = q.x;
     a.qy = q.y;
-    a.zbh = z.x;
-    a.zbl = z.y;
     a.mult = X.P->melt_mult ? at_off(X.P->melt_mult, o4) : 1.0f;
     if (p == X.inflow_p) a.f |= PF_INFLOW;
 }
@@ -397,8 +394,7 @@ __device__ inline void store_col(const sweep_ctx &X, ring &R, const pref &a, int
     R.u[SL][i] = qx * den;
     R.v[par][i] = qy * den;
     R.sq[par][i] = fx_sqrt(h);
-    R.zbh[par][i] = inner ? a.zbh : 0.f;
-    R.zbl[par][i] = inner ? a.zbl : 0.f;
+    R.zb[par][i] = inner ? a.zb : 0.0;
     R.fl[SL][i] = (uint8_t)((wet ? LW : 0) | ((a.f & FXF_WEIR) ? LX : 0) | (inner ? LI : 0) |
                             ((a.f & FXF_STALE) ? LS : 0));
     R.raw[par][i] = (uint8_t)a.f;
@@ -420,14 +416,16 @@ __device__ inline void flux_col(const sweep_ctx &X, ring &R, int c)
             if (!wetW && !wetE && !wetS && !wetN) {
                 if (fl & LS) {                     // stale fluxes of an all-dry neighbourhood
                     const uint32_t p = (uint32_t)X.row + (uint32_t)c * X.ld;
-                    for (int m = 0; m < 6; m++) f[m] = at_off(X.S->f6 + m * X.S->mm, p * 4u);
+                    const float2 *s6 = reinterpret_cast<const float2 *>(X.S->f6 + (size_t)p * 6);
+                    const float2 a = s6[0], b = s6[1], e = s6[2];
+                    f[0] = a.x; f[1] = a.y; f[2] = b.x; f[3] = b.y; f[4] = e.x; f[5] = e.y;
                 }
             } else {
                 solver_dry32(X.K, front_of(R, pc, pn, i), wetE, wetN, X.dxy_over_dt, f);
             }
         } else {
-            const float dzbE = (R.zbh[pc][i + 1] - R.zbh[pc][i]) + (R.zbl[pc][i + 1] - R.zbl[pc][i]);
-            const float dzbN = (R.zbh[pn][i] - R.zbh[pc][i]) + (R.zbl[pn][i] - R.zbl[pc][i]);
+            const float dzbE = (float)(R.zb[pc][i + 1] - R.zb[pc][i]);
+            const float dzbN = (float)(R.zb[pn][i] - R.zb[pc][i]);
             front64 fr;
             if (!wetE || !wetN) fr = front_of(R, pc, pn, i);
             nb32 n;
@@ -513,7 +511,10 @@ __device__ inline void update_col(const sweep_ctx &X, ring &R, int c, double &dt
                 bool nz = false;
                 for (int m = 0; m < 6; m++) nz |= rflux(R, fs, m, i) != 0.f;
                 if (nz) {
-                    for (int m = 0; m < 6; m++) at_off(S.f6 + m * S.mm, o4) = rflux(R, fs, m, i);
+                    float2 *s6 = reinterpret_cast<float2 *>(S.f6 + (size_t)p * 6);
+                    s6[0] = make_float2(rflux(R, fs, 0, i), rflux(R, fs, 1, i));
+                    s6[1] = make_float2(rflux(R, fs, 2, i), rflux(R, fs, 3, i));
+                    s6[2] = make_float2(rflux(R, fs, 4, i), rflux(R, fs, 5, i));
                     nf |= FXF_STALE;
                 } else {
                     nf &= ~FXF_STALE;
@@ -523,10 +524,11 @@ __device__ inline void update_col(const sweep_ctx &X, ring &R, int c, double &dt
         // wet-time tracer: runs are opened / closed on transitions only
         if (wet1 != (bool)(f & FXF_WETPREV)) {
             if (wet1) {
-                at_off(S.tw_t0, o4) = (float)X.s_wet_prev;
+                at_off(reinterpret_cast<float *>(S.tw), p * 8u + 4u) = (float)X.s_wet_prev;
                 nf |= FXF_WETPREV;
             } else {
-                at_off(S.tw_acc, o4) += (float)(X.s_wet_prev - (double)at_off(S.tw_t0, o4));
+                const float2 w = at_off(S.tw, p * 8u);
+                at_off(reinterpret_cast<float *>(S.tw), p * 8u) = w.x + (float)(X.s_wet_prev - (double)w.y);
                 nf &= ~FXF_WETPREV;
             }
         }
@@ -540,8 +542,7 @@ __device__ inline void update_col(const sweep_ctx &X, ring &R, int c, double &dt
     at_off(X.d_out, p * 8u) = d;
     at_off(X.q_out, p * 8u) = make_float2(qxn, qyn);
     if (X.ade_on) {
-        at_off(S.ff, o4) = fe1P;
-        at_off(S.ff + S.mm, o4) = fn1P;
+        at_off(S.ff, p * 8u) = make_float2(fe1P, fn1P);
     }
     if (nf != f) at_off(S.flags, p) = nf;
     cfl_cell(P, d, qxn, qyn, dtm, hmx);
@@ -956,12 +957,11 @@ s32p make_s32p(fx_domain d)
     s32p S;
     S.d_in[0] = t.d[0]; S.d_in[1] = t.d[1];
     S.q_in[0] = t.q[0]; S.q_in[1] = t.q[1];
-    S.zb2 = t.zb2;
+    S.zb = t.zb;
     S.f6 = t.f6;
     S.ff = t.ff;
     S.flags = t.flags;
-    S.tw_acc = t.tw_acc;
-    S.tw_t0 = t.tw_t0;
+    S.tw = t.tw;
     S.partials = t.partials;
     S.mm = d->ncell;
     return S;
@@ -1025,12 +1025,11 @@ int fx32_init_state(fx_domain d)
         if ((rc = dalloc32(d, &t.d[k], mm))) return rc;
         if ((rc = dalloc32(d, &t.q[k], mm))) return rc;
     }
-    if ((rc = dalloc32(d, &t.zb2, mm))) return rc;
+    if ((rc = dalloc32(d, &t.zb, mm))) return rc;
     if ((rc = dalloc32(d, &t.f6, 6 * mm))) return rc;
-    if ((rc = dalloc32(d, &t.ff, 2 * mm))) return rc;
+    if ((rc = dalloc32(d, &t.ff, mm))) return rc;
     if ((rc = dalloc32(d, &t.flags, mm))) return rc;
-    if ((rc = dalloc32(d, &t.tw_acc, mm))) return rc;
-    if ((rc = dalloc32(d, &t.tw_t0, mm))) return rc;
+    if ((rc = dalloc32(d, &t.tw, mm))) return rc;
     if (d->desc.ade_solver)
         for (int b = 0; b < 2; b++)
             for (int k = 0; k < d->desc.nchem; k++)
@@ -1055,12 +1054,7 @@ int fx32_set_field(fx_domain d, int field, const void *host)
     if ((rc = get_cur(d, &cur))) return rc;
     if (field == FX_F_ZB) {          // from fx_set_static: FP32 bed, flags base
         const double *zb = (const double *)host;
-        std::vector<float2> z2(mm);
-        for (int64_t i = 0; i < mm; i++) {
-            z2[i].x = (float)zb[i];
-            z2[i].y = (float)(zb[i] - (double)z2[i].x);
-        }
-        FX_HIP(hipMemcpy(t.zb2, z2.data(), mm * 8, hipMemcpyHostToDevice));
+        FX_HIP(hipMemcpy(t.zb, zb, mm * 8, hipMemcpyHostToDevice));
         return FX_OK;
     }
     if (field == -1) {               // weir mask (uint8 0/1) -> flag bytes
@@ -1104,9 +1098,11 @@ int fx32_set_field(fx_domain d, int field, const void *host)
     }
     case FX_F_FE1: case FX_F_FE2: case FX_F_FE3: case FX_F_FN1: case FX_F_FN2: case FX_F_FN3: {
         const double *q = (const double *)host;
-        std::vector<float> qf(mm);
-        for (int64_t i = 0; i < mm; i++) qf[i] = (float)q[i];
-        FX_HIP(hipMemcpy(t.f6 + (int64_t)(field - FX_F_FE1) * mm, qf.data(), mm * 4, hipMemcpyHostToDevice));
+        const int m = field - FX_F_FE1;
+        std::vector<float> f6(6 * mm), qf(mm);
+        FX_HIP(hipMemcpy(f6.data(), t.f6, 6 * mm * 4, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < mm; i++) f6[6 * i + m] = qf[i] = (float)q[i];
+        FX_HIP(hipMemcpy(t.f6, f6.data(), 6 * mm * 4, hipMemcpyHostToDevice));
         // stored-flux flag bit
         std::vector<uint8_t> fl(mm);
         FX_HIP(hipMemcpy(fl.data(), t.flags, mm, hipMemcpyDeviceToHost));
@@ -1117,9 +1113,9 @@ int fx32_set_field(fx_domain d, int field, const void *host)
     }
     case FX_F_TWET: {
         const double *q = (const double *)host;
-        std::vector<float> qf(mm);
-        for (int64_t i = 0; i < mm; i++) qf[i] = (float)q[i];
-        FX_HIP(hipMemcpy(t.tw_acc, qf.data(), mm * 4, hipMemcpyHostToDevice));
+        std::vector<float2> w(mm);
+        for (int64_t i = 0; i < mm; i++) w[i] = make_float2((float)q[i], 0.0f);
+        FX_HIP(hipMemcpy(t.tw, w.data(), mm * 8, hipMemcpyHostToDevice));
         std::vector<uint8_t> fl(mm);
         FX_HIP(hipMemcpy(fl.data(), t.flags, mm, hipMemcpyDeviceToHost));
         for (auto &f : fl) f &= ~FXF_WETPREV;
@@ -1218,19 +1214,19 @@ int fx32_get_field(fx_domain d, int field, void *host)
         return FX_OK;
     }
     case FX_F_FE1: case FX_F_FE2: case FX_F_FE3: case FX_F_FN1: case FX_F_FN2: case FX_F_FN3: {
-        std::vector<float> q(mm);
-        FX_HIP(hipMemcpy(q.data(), t.f6 + (int64_t)(field - FX_F_FE1) * mm, mm * 4, hipMemcpyDeviceToHost));
-        for (int64_t i = 0; i < mm; i++) out[i] = q[i];
+        const int m = field - FX_F_FE1;
+        std::vector<float> f6(6 * mm);
+        FX_HIP(hipMemcpy(f6.data(), t.f6, 6 * mm * 4, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < mm; i++) out[i] = f6[6 * i + m];
         return FX_OK;
     }
     case FX_F_TWET: {
         if ((rc = load_fl())) return rc;
-        std::vector<float> acc(mm), t0(mm);
-        FX_HIP(hipMemcpy(acc.data(), t.tw_acc, mm * 4, hipMemcpyDeviceToHost));
-        FX_HIP(hipMemcpy(t0.data(), t.tw_t0, mm * 4, hipMemcpyDeviceToHost));
+        std::vector<float2> w(mm);
+        FX_HIP(hipMemcpy(w.data(), t.tw, mm * 8, hipMemcpyDeviceToHost));
         const double s = d->ctl_host->s_wet;
         for (int64_t i = 0; i < mm; i++)
-            out[i] = acc[i] + ((fl[i] & FXF_WETPREV) ? s - (double)t0[i] : 0.0);
+            out[i] = w[i].x + ((fl[i] & FXF_WETPREV) ? s - (double)w[i].y : 0.0);
         return FX_OK;
     }
     default:

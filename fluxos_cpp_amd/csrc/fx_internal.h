@@ -71,13 +71,12 @@ struct fx_state64 {
 struct fx_state32 {
     double *d[2];              // z - zb, signed (the reference keeps deficits)
     float2 *q[2];              // (qx, qy) of each cell, one 8-byte load
-    float2 *zb2;               // bed elevation: FP32 high part, and the low part
-                               // zb - (double)high, so bed differences are exact to ~1e-12 m
-    float *f6;                 // persistent fluxes fe1,fe2,fe3,fn1,fn2,fn3 (S1), SoA planes
-    float *ff;                 // fe1 / fn1 of the step for the fast ADE (two planes)
+    double *zb;                // bed elevation (FP64: bed differences to ~1e-13 m)
+    float *f6;                 // persistent fluxes fe1,fe2,fe3,fn1,fn2,fn3 (S1), six per cell
+    float2 *ff;                // (fe1, fn1) of the step for the fast ADE
     float *mass[2][FX_MAX_CHEM];  // solute mass per unit area (c*h), ping-pong
     uint8_t *flags;            // FXF_* bits
-    float *tw_acc, *tw_t0;     // wet-time tracer: closed runs, start of the open run
+    float2 *tw;                // wet-time tracer: (closed runs, start of the open run)
     double *partials;          // 3 per step-kernel block (dt, hpall, weir volume)
     double *part2;             // 3 per clock block (second level of the fold)
     uint32_t *ticket;          // last-block counter of the clock

@@ -8,11 +8,11 @@
 struct s32p {
     const double *d_in[2];
     const float2 *q_in[2];          // (qx, qy)
-    const float2 *zb2;              // bed high and low parts
+    const double *zb;               // bed elevation
     float *f6;
-    float *ff;
+    float2 *ff;                     // (fe1, fn1) of the step
     uint8_t *flags;
-    float *tw_acc, *tw_t0;
+    float2 *tw;                     // wet-time tracer (closed runs, open-run start)
     double *partials;
     int64_t mm;
 };
@@ -20,7 +20,7 @@ struct s32p {
 // pointers of one domain's state as the fast ADE sweep sees them
 struct a32p {
     const double *d[2];
-    const float *ff;
+    const float2 *ff;
     const uint8_t *flags;
     float *mass[2][FX_MAX_CHEM];
     const float *melt_mult;
