@@ -7,12 +7,15 @@ against the number of waves in flight.  Results do not depend on it (every
 cell is computed by one lane from the same inputs either way).
 
 ``tune(n)`` times every candidate on a synthetic basin of side n (the
-fractal-DEM generator, two solutes) spun up on the device to t = 7200 s and
-then run on, candidate after candidate, and keeps the fastest for each kernel; ``main`` tunes a ladder of
-sizes and writes ``tuning.json`` beside this file, which
-``HipDomain.from_scenario`` consults (nearest size by cell count).
+fractal-DEM generator, two solutes) spun up on the device to t = 3600 s (the
+melt has wetted the basin) and then run on, candidate after candidate, and
+keeps the fastest for each kernel; ``main`` tunes a ladder of sizes and writes
+``tuning.json`` beside this file, which ``HipDomain.from_scenario`` consults
+(nearest size by cell count).  The time per block count is not smooth in
+``ch``: blocks run in rounds of the device's wave slots, and a last round
+that is nearly empty costs as much as a full one.
 
-    python -m fluxos_cpp_amd.autotune [--sizes 512,1024,2048,4096] [--steps 40]
+    python -m fluxos_cpp_amd.autotune [--sizes 512,1024,2048,4096,8192,16384] [--steps 40]
 """
 from __future__ import annotations
 
@@ -25,7 +28,7 @@ from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 TABLE = PKG / "tuning.json"
-CANDIDATES_CH = (8, 12, 16, 24, 32, 64)
+CANDIDATES_CH = (8, 12, 16, 20, 24, 28, 32, 40, 48, 64)
 CANDIDATES_ACH = (16, 32, 48, 64, 96, 128)
 FX32_ACH_DEFAULT = 64
 
@@ -44,7 +47,7 @@ def _kernel_us(dom, steps: int) -> dict[str, float]:
     return out
 
 
-def tune(n: int, steps: int = 40, t_wet: float = 7200.0) -> dict:
+def tune(n: int, steps: int = 40, t_wet: float = 3600.0) -> dict:
     from . import capi, gen
     sc = gen.scenario_c2(n)
     init = capi.prepare(sc)
@@ -108,7 +111,7 @@ def choose(nrows: int, ncols: int, path: Path = TABLE) -> tuple[int, int] | None
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--sizes", default="512,1024,2048,4096")
+    ap.add_argument("--sizes", default="512,1024,2048,4096,8192,16384")
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--out", default=str(TABLE))
     args = ap.parse_args(argv)
