@@ -83,6 +83,25 @@ def tune(n: int, steps: int = 40, t_wet: float = 3600.0) -> dict:
             "workload": f"fractal {n}x{n}, 2 solutes, run on from t={t_wet:.0f}s, {steps} steps per shape, best of 2"}
 
 
+BATCH_CANDIDATES = ((8, 16), (8, 32), (8, 64), (12, 32), (16, 32), (20, 16))
+
+
+def tune_batch(count: int = 64, candidates=BATCH_CANDIDATES) -> dict:
+    """Tile shape for batches: the ensemble of `count` C3 scenarios (sides
+    256-2048) run whole under each candidate; the fastest by cell-updates/s.
+    Small members weigh more here than in any single-domain entry."""
+    from . import capi, gen, serve
+    scens = [gen.scenario_c3(i) for i in range(count)]
+    inits = [capi.prepare(s) for s in scens]
+    rates = {}
+    for t in candidates:
+        rates[f"{t[0]},{t[1]}"] = serve.run_ensemble(scens, inits, tiling=t)["cell_updates_per_s"]
+    best = max(rates, key=rates.get)
+    ch, ach = (int(x) for x in best.split(","))
+    return {"ch": ch, "ach": ach, "cell_updates_per_s": {k: round(v, 1) for k, v in rates.items()},
+            "workload": f"{count} C3 scenarios in one batch, 1 simulated hour each"}
+
+
 def load_table(path: Path = TABLE) -> list[dict]:
     if not path.exists():
         return []
@@ -104,6 +123,19 @@ def choose_cells(cells: int, path: Path = TABLE) -> tuple[int, int] | None:
     return int(e["ch"]), int(e["ach"])
 
 
+def choose_batch(member_cells: list[int], path: Path = TABLE) -> tuple[int, int] | None:
+    """(ch, ach) for the members of a batch: the table's "batch" entry when it
+    has one (tuned on an ensemble), else the entry nearest the batch's total
+    cell count (how full the device is)."""
+    try:
+        b = json.loads(Path(path).read_text()).get("batch") if Path(path).exists() else None
+    except (ValueError, OSError):
+        b = None
+    if b:
+        return int(b["ch"]), int(b["ach"])
+    return choose_cells(sum(member_cells), path)
+
+
 def choose(nrows: int, ncols: int, path: Path = TABLE) -> tuple[int, int] | None:
     """(ch, ach) for a single domain of nrows x ncols."""
     return choose_cells(nrows * ncols, path)
@@ -114,6 +146,7 @@ def main(argv=None) -> int:
     ap.add_argument("--sizes", default="512,1024,2048,4096,8192,16384")
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--out", default=str(TABLE))
+    ap.add_argument("--batch", action="store_true", help="also tune the batch entry (about a minute per candidate)")
     args = ap.parse_args(argv)
     from . import capi
     capi.load()
@@ -124,7 +157,18 @@ def main(argv=None) -> int:
         entries.append(e)
         print(f"[autotune] n={n}: ch={e['ch']} ach={e['ach']} sweep {e['k32_sweep_us']} ade {e['k32_ade_us']} "
               f"({time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
-    Path(args.out).write_text(json.dumps({"entries": entries}, indent=1) + "\n")
+    table = {"entries": entries}
+    if args.batch:
+        table["batch"] = tune_batch()
+        print(f"[autotune] batch: {table['batch']}", file=sys.stderr, flush=True)
+    else:
+        try:                                 # keep the batch entry already tuned
+            old = json.loads(Path(args.out).read_text()).get("batch")
+        except (ValueError, OSError):
+            old = None
+        if old:
+            table["batch"] = old
+    Path(args.out).write_text(json.dumps(table, indent=1) + "\n")
     return 0
 
 

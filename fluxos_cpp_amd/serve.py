@@ -86,14 +86,16 @@ def _drive(batch: capi.Batch, t_end: dict, next_print: dict, print_step: float, 
 
 
 def run_ensemble(scens: list[gen.Scenario], inits: list[capi.InitialState], ade: bool = True,
-                 steps_per_poll: int = 16) -> dict:
+                 steps_per_poll: int = 16, tiling: tuple[int, int] | None = None) -> dict:
     """Admit every scenario into one batch and run all to their end time.
-    The timed region starts with every input resident on the device."""
+    The timed region starts with every input resident on the device.  The
+    members' tile shape is `tiling`, or the autotuner's batch entry."""
     from . import autotune
     nchem = max(sc.nchem for sc in scens)
     batch = capi.Batch(nchem=nchem, ade=ade, max_members=len(scens))
     doms, t_end, next_print = {}, {}, {}
-    tiling = autotune.choose_cells(sum(sc.nrows * sc.ncols for sc in scens))
+    if tiling is None:
+        tiling = autotune.choose_batch([sc.nrows * sc.ncols for sc in scens])
     for sc, st in zip(scens, inits):
         d = capi.HipDomain.from_scenario(sc, precision=capi.FP32_MIXED, ade=ade, init=st, tiling=tiling)
         s = batch.admit(d, float(sc.t_end), float(sc.print_step))
@@ -257,9 +259,9 @@ def serve(requests: list[Request], budget_bytes: int = 200 * 10 ** 9, max_member
                     break
                 pending.popleft()
                 # tile shape for how full the device will be with this member
-                live_cells = sum(r.cells for r, _, _ in live.values()) + req.cells
+                sizes = [r.cells for r, _, _ in live.values()] + [req.cells]
                 dom = capi.HipDomain.from_scenario(req.scenario, precision=capi.FP32_MIXED, ade=True,
-                                                   init=req.init, tiling=autotune.choose_cells(live_cells))
+                                                   init=req.init, tiling=autotune.choose_batch(sizes))
                 try:
                     if prepare_hook is not None:
                         prepare_hook(req, dom)

@@ -38,6 +38,24 @@ def test_tile_table_missing_or_corrupt(tmp_path):
     assert autotune.choose(256, 256, bad) is None
 
 
+def test_batch_entry_wins_for_batches_else_nearest_total(tmp_path):
+    entries = [{"n": 512, "cells": 512 ** 2, "ch": 8, "ach": 16},
+               {"n": 4096, "cells": 4096 ** 2, "ch": 24, "ach": 64}]
+    p = _table(tmp_path, entries)
+    # no batch entry: the batch's total cell count picks the size entry
+    assert autotune.choose_batch([256 ** 2] * 4, p) == (8, 16)
+    assert autotune.choose_batch([2048 ** 2] * 4, p) == (24, 64)
+    p.write_text(json.dumps({"entries": entries, "batch": {"ch": 12, "ach": 32}}))
+    assert autotune.choose_batch([2048 ** 2] * 4, p) == (12, 32)
+    assert autotune.choose(4096, 4096, p) == (24, 64)         # single domains ignore it
+
+
+def test_committed_batch_entry_is_the_fastest_measured():
+    b = json.loads(autotune.TABLE.read_text())["batch"]
+    rates = b["cell_updates_per_s"]
+    assert max(rates, key=rates.get) == f"{b['ch']},{b['ach']}"
+
+
 def test_committed_table_is_well_formed():
     entries = autotune.load_table()
     assert entries, "fluxos_cpp_amd/tuning.json is missing or empty"
