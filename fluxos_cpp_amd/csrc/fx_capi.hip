@@ -367,8 +367,8 @@ int fx_domain_create(const fx_domain_desc *desc, fx_domain *out)
             if ((rc = dalloc(d, &s.conc_tmp, mm * d->desc.nchem))) return bail(rc);
             d->ade_blocks = (d->desc.ncols + 63) / 64;
             const int64_t nb = (int64_t)d->ade_blocks * d->desc.nchem;
-            const int64_t words = ((nb + 63) / 64) * 64 + 2 * nb * d->mrows;
-            if ((rc = dalloc(d, &d->ade_progress, words))) return bail(rc);
+            // edge words (north-face flux, final value) of every block's last column
+            if ((rc = dalloc(d, &d->ade_edges, 2 * nb * d->mrows))) return bail(rc);
         }
         d->bytes = fp64_bytes(d->desc);
     } else {
@@ -402,7 +402,7 @@ int fx_domain_destroy(fx_domain d)
     dfree(d->soil);
     dfree(d->ctl);
     dfree(d->partials);
-    dfree(d->ade_progress);
+    dfree(d->ade_edges);
     dfree(d->meteo);
     dfree(d->melt_mult);
     dfree(d->inflow);

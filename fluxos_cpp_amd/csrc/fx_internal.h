@@ -112,7 +112,7 @@ struct fx_domain_s {
     fx_ctl *ctl_host;          // pinned mirror
     double *partials;          // CFL block partials (2 per block)
     int32_t npartials;
-    uint32_t *ade_progress;    // per-block progress of the ADE wavefront
+    unsigned long long *ade_edges;   // exact ADE: last-column edge words of every block
     int32_t ade_blocks;
     float *meteo; int32_t meteo_nrows, meteo_ncols;
     float *melt_mult;
