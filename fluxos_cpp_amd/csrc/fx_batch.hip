@@ -273,6 +273,7 @@ int fx_batch_admit(fx_batch b, fx_domain d, double t_end, double print_next, int
     b->nab[s] = fx32_ade_blocks(d);
     b->member[s] = d;
     d->in_batch = true;
+    d->cfl_fresh = false;               // the batch steps it
     b->reported[s] = 0;
     b->t_end[s] = t_end;
     b->stat_host[s] = fx_bstat{c->tim, c->dtfl, c->it, 0, 0, c->halt, c->status, 0};

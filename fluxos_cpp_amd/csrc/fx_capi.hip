@@ -205,6 +205,7 @@ int enter_alone(fx_domain d)
     int rc = enter(d);
     if (rc) return rc;
     if (d->in_batch) return fx_fail(FX_ERR_STATE, "domain is a batch member: evict it first");
+    d->cfl_fresh = false;                   // the call may change the state
     return FX_OK;
 }
 
@@ -340,6 +341,7 @@ int fx_domain_create(const fx_domain_desc *desc, fx_domain *out)
     d->ch = FX32_CH;
     d->ach = FX32_ACH;
     d->in_batch = false;
+    d->cfl_fresh = false;
     auto bail = [&](int code) { fx_domain_destroy(d); return code; };
     if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fx_fail(FX_ERR_HIP, "hipStreamCreate failed"));
@@ -524,6 +526,7 @@ int fx_set_wintra(fx_domain d, int32_t status, double soil_release_rate)
 {
     int rc = enter(d);
     if (rc) return rc;
+    d->cfl_fresh = false;
     if (d->in_batch) return fx_fail(FX_ERR_STATE, "domain is a batch member");
     if (!std::isfinite(soil_release_rate)) return fx_fail(FX_ERR_ARG, "soil_release_rate must be finite");
     if (status) {
@@ -716,6 +719,10 @@ int fx_adesolver_calc(fx_domain d, int32_t it, int32_t ichem)
 
 int fx_advance(fx_domain d, double t_end, double print_next, int32_t max_steps, fx_step_info *info)
 {
+    // FP32: when nothing changed the state since the last call's last sweep,
+    // that sweep's CFL partials are the current state's, and the first
+    // iteration's clock folds them as any later iteration's does
+    const bool fresh = d && d->cfl_fresh;
     int rc = enter_alone(d);
     if (rc) return rc;
     if (!d->has_static) return fx_fail(FX_ERR_STATE, "fx_set_static has not been called");
@@ -732,8 +739,10 @@ int fx_advance(fx_domain d, double t_end, double print_next, int32_t max_steps, 
         return FX_OK;
     }
     if ((rc = fx_inflow_cell_ok(d))) return rc;
-    if (d->desc.precision == FX_FP32_MIXED && (rc = fx32_launch_cfl_initial(d))) return rc;
-    if (d->desc.precision == FX_FP32_MIXED && (rc = pull_ctl(d))) return rc;
+    if (d->desc.precision == FX_FP32_MIXED && !fresh) {
+        if ((rc = fx32_launch_cfl_initial(d))) return rc;
+        if ((rc = pull_ctl(d))) return rc;
+    }
     c->print_next = print_next;
     c->t_end = t_end;
     c->halt = 0;
@@ -776,6 +785,7 @@ int fx_advance(fx_domain d, double t_end, double print_next, int32_t max_steps, 
         return fx_fail(FX_ERR_ARG, "inflow ix,iy in NODATA_VALUE cell");
     }
     if (c->status & 4) return fx_fail(FX_ERR_NONFINITE, "non-finite value in the state");
+    d->cfl_fresh = d->desc.precision == FX_FP32_MIXED;
     return FX_OK;
 }
 
@@ -783,6 +793,7 @@ int fx_set_tiling(fx_domain d, int32_t ch, int32_t ach)
 {
     int rc = enter(d);
     if (rc) return rc;
+    d->cfl_fresh = false;                   // the partials follow the tiles
     if (d->desc.precision != FX_FP32_MIXED) return fx_fail(FX_ERR_UNSUPPORTED, "tiling applies to FP32-mixed domains");
     if (d->in_batch) return fx_fail(FX_ERR_STATE, "domain is a batch member: its tiling is fixed");
     if (ch < FX32_CH_MIN || ch > 4096 || ach < FX32_CH_MIN || ach > 4096)

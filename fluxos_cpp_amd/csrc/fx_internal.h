@@ -130,6 +130,7 @@ struct fx_domain_s {
     bool graph_ok;
     int32_t ch, ach;           // FP32 sweep column-block lengths (fx_set_tiling)
     bool in_batch;             // member of a batch: tiling is fixed
+    bool cfl_fresh;            // FP32: the CFL partials are the last sweep's, of the current state
     bool wintra;               // WINTRA soil release on (fx_set_wintra)
     double soil_rate;          // SOIL_RELEASE_RATE [1/h]
     double *soil;              // soil_mass grid (FP64, both precisions); NULL until needed
