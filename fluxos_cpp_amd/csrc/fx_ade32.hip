@@ -54,7 +54,7 @@ struct apref {
 
 // per-cell geometry of one column, shared by the solutes
 struct ageo {
-    float h, fe, fn;
+    float h, sh, fe, fn;      // depth, its square root, the step's face discharges
     bool w;
 };
 
@@ -117,6 +117,7 @@ __device__ inline void a_derive(const actx &X, const apref<NC> &a, ageo &g, floa
     const float h0 = inner ? (float)fmax(0.0, dm) : 0.f;
     const float mlt = (inner && a.dold > 0.0) ? (float)melt : 0.f;   // solute arrives with melt only into water
     g.h = inner ? (float)hn : 0.f;
+    g.sh = fx_sqrt(g.h);
     g.w = w;
     g.fe = inner ? a.fe : 0.f;
     g.fn = inner ? a.fn : 0.f;
@@ -145,10 +146,14 @@ __device__ inline float a_face(float q, float cp, float cq, bool wq, float cu, b
     const float pd = -(sq * kdiff) * (cq - cp);
     // upwind-biased face value: -1/8 of the far upstream value, 3/4 of the
     // upstream one and 3/8 of the downstream one, upstream being P for
-    // q > 0 and Q otherwise; the plain mean where the far value is dry
+    // q > 0 and Q otherwise; the plain mean where the far value is dry.
+    // Written as the mean less 1/8 of the upwind curvature far - 2 up + down,
+    // where up + down = cp + cq whichever way the flow goes.
     const bool pos = q > 0.f;
-    const float far = pos ? cu : cd, up = pos ? cp : cq, down = pos ? cq : cp;
-    const float fm = fx_max0((pos ? wu : wd) ? -.125f * far + .75f * up + .375f * down : .5f * (cp + cq));
+    const float far = pos ? cu : cd, up = pos ? cp : cq;
+    const float sum = cp + cq;
+    const float curv = (pos ? wu : wd) ? (far + sum) - 3.f * up : 0.f;
+    const float fm = fx_max0(.5f * sum - .125f * curv);
     float pf = q * fm * dx + pd;
     if (pf < 0.f) pf = fmaxf(pf, cq * cap);
     return pf;
@@ -213,8 +218,10 @@ template <int NC>
 __device__ inline void a_column(const actx &X, awin<NC> &W, int c, bool upd, bool own, bool &bad)
 {
     const ageo &gP = W.g[1], &gN = W.g[2], &gNN = W.g[3], &gS = W.g[0];
-    const bool wW = __shfl_up((int)gP.w, 1, 64), wE = __shfl_down((int)gP.w, 1, 64);
-    const bool wEE = __shfl_down((int)gP.w, 2, 64);
+    // the wet flags of rows r-1, r+1 and r+2 from the wave's mask of column c
+    // (an edge lane sees a dry row beyond the wave; it owns no cell)
+    const uint64_t mw = __builtin_amdgcn_ballot_w64(gP.w);
+    const bool wW = lane_bit(mw << 1), wE = lane_bit(mw >> 1), wEE = lane_bit(mw >> 2);
     const float hE = dn1(gP.h);
     const uint32_t p = (uint32_t)X.row + (uint32_t)c * X.ld;
     const bool act = X.transport && gP.w;
@@ -222,8 +229,10 @@ __device__ inline void a_column(const actx &X, awin<NC> &W, int c, bool upd, boo
     // floors, h*area/dt of the limiter and dt/(area*h) of the update
     float sE = 0.f, sN = 0.f, hA = 0.f, kup = 0.f;
     if (act) {
-        sE = fx_sqrt(fx_max_nn(gP.h * hE, 1e-4f));
-        sN = fx_sqrt(fx_max_nn(gP.h * gN.h, 1e-4f));
+        // sqrt(max(hp hq, 1e-4)) as max(sqrt(hp) sqrt(hq), 0.01): one square
+        // root per cell instead of one per face
+        sE = fx_max_nn(gP.sh * dn1(gP.sh), .01f);
+        sN = fx_max_nn(gP.sh * gN.sh, .01f);
         hA = gP.h * X.area_dt;
         kup = X.dt_area * __builtin_amdgcn_rcpf(gP.h);
     }

@@ -182,6 +182,9 @@ int check_desc(const fx_domain_desc *D)
         return fx_fail(FX_ERR_ARG, "unknown precision");
     if (D->precision == FX_FP32_MIXED && (int64_t)(D->nrows + 2) * (D->ncols + 2) >= FX32_MAX_CELLS)
         return fx_fail(FX_ERR_ARG, "FP32-mixed domains hold fewer than 2^29 cells (ghost rows included)");
+    // the FP32 kernels take every depth above hdry as a normal FP32 number
+    if (D->precision == FX_FP32_MIXED && !(D->hdry >= 1e-30))
+        return fx_fail(FX_ERR_ARG, "FP32-mixed domains need ROUGNESS_HEIGHT (hdry) >= 1e-30 m");
     if (D->ade_mode < FX_ADE_OFF || D->ade_mode > FX_ADE_FAST) return fx_fail(FX_ERR_ARG, "unknown ade_mode");
     if (D->precision == FX_FP64_PARITY && D->ade_solver && D->ade_mode == FX_ADE_FAST)
         return fx_fail(FX_ERR_UNSUPPORTED, "FP64 parity domains run the exact ADE only");

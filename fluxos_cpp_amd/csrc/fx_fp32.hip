@@ -37,14 +37,128 @@
 
 namespace {
 
-// LDS flag bits
-#define LW 1u   // wet in loop 1
-#define LX 2u   // weir
-#define LI 4u   // interior
-#define LS 8u   // stored fluxes nonzero
+// wave reductions, and the clock of one loop iteration (main.cpp:271-325,
+// 385-387) from the folded CFL partials of the previous step sweep: used by
+// k32_clock (one domain) and k32b_clock (batch members)
+__device__ inline double wmin(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ inline double wmax(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ inline double wsum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// the CFL time step from the largest wave speed of the domain (9e10 s, the
+// reference's initial bound, where no cell is above hdry)
+__device__ inline double cfl_dt(const fx_params &P, double smax)
+{
+    return smax > 0.0 ? (double)(P.cfl_dxy * fx_rcp((float)smax)) : 9.e10;
+}
+
+__device__ inline void kahan_add(double &sum, double &comp, double v)
+{
+    const double y = v - comp;
+    const double t = sum + y;
+    comp = (t - sum) - y;
+    sum = t;
+}
+
+__device__ inline void clock_advance(const fx_params &P, fx_ctl *ctl, double a, double b, double c, double m)
+{
+    // Kahan-compensated ledger of the weir-face exchange of the previous step
+    kahan_add(ctl->led_weir, ctl->led_weir_c, c * P.arbase);
+    // solute the previous step's melt brought into water (m: melt area of
+    // the cells that held water; meteoi and cmelt are still that step's)
+    if (P.meteo && P.ade)
+        for (int k = 0; k < P.nchem; k++)
+            kahan_add(ctl->sled[k], ctl->sled_c[k], ctl->meteoi * m * P.arbase * ctl->cmelt[k]);
+    const double dtc = cfl_dt(P, a);                   // a: the largest wave speed
+    ctl->dt_raw = dtc;
+    const double dt = fmin(ctl->print_next - ctl->tim, dtc);
+    ctl->dtfl = dt;
+    ctl->hpall = b;
+    ctl->tim = ctl->tim + dt;
+    const int gate = (b != 0.0);
+    ctl->gate = gate;
+    ctl->it += gate;
+    ctl->steps += 1;
+    ctl->solver_steps += gate;
+    ctl->k_dt_dxy = (float)(dt / P.dxy);
+    ctl->k_dt = (float)dt;
+    ctl->k_half_dx_dt = (float)(0.5 * P.dxy / dt);
+    ctl->k_arbase_dt = (float)(P.arbase / dt);
+    ctl->k_dxy_dt = (float)(P.dxy / dt);
+    ctl->s_wet_prev = ctl->s_wet;
+    if (gate) ctl->s_wet = ctl->s_wet + dt / 3600;
+}
+
+__device__ inline void clock_forcing(const fx_params &P, fx_ctl *ctl, int mrow, int irow)
+{
+    fx_forcing_eval(P, ctl, mrow, irow);
+    if (P.inflow) {                  // the inflow cell's bed is never NODATA here (checked by the host)
+        // water in: the source, plus a deficit of the cell filled up to the
+        // bed first (common.cpp:324-325), judged after this step's melt
+        double vin = ctl->inflowi;
+        if (P.d32[0]) {
+            const size_t p = (size_t)P.inflow_p;
+            double dm = P.d32[ctl->cur][p];
+            if (P.meteo) dm += ctl->meteoi * (P.melt_mult ? (double)P.melt_mult[p] : 1.0);
+            if (dm <= 0.0) vin -= dm;
+            // its solute enters only water (common.cpp:350-361)
+            if (dm > 0.0 && P.ade)
+                for (int k = 0; k < P.nchem; k++)
+                    kahan_add(ctl->sled[k], ctl->sled_c[k], ctl->inflowi * P.arbase * ctl->cin[k]);
+        }
+        const double y = vin * P.arbase - ctl->led_melt_c;
+        const double t = ctl->led_melt + y;
+        ctl->led_melt_c = (t - ctl->led_melt) - y;
+        ctl->led_melt = t;
+    }
+    if (P.meteo) {
+        const double y = ctl->meteoi * ctl->melt_area * P.arbase - ctl->led_melt_c;
+        const double t = ctl->led_melt + y;
+        ctl->led_melt_c = (t - ctl->led_melt) - y;
+        ctl->led_melt = t;
+    }
+    ctl->pending_halt = (ctl->tim >= ctl->print_next);
+}
+
+// clock_finish: the step's clock from the folded (wave speed, hpall, weir).  Every
+// thread of the block calls it (the forcing-table lookup is shared); thread 0
+// holds the folded values and writes the control block.
+__device__ inline void clock_finish(const fx_params &P, fx_ctl *ctl, double a, double b, double c, double m)
+{
+    __shared__ double tim_new;
+    if (threadIdx.x == 0) clock_advance(P, ctl, a, b, c, m);
+    if (threadIdx.x == 0) tim_new = ctl->tim;
+    __syncthreads();
+    const int mrow = P.meteo ? fx_forcing_row_block(P.meteo, P.meteo_nrows, P.meteo_ncols, tim_new) : -1;
+    const int irow = P.inflow ? fx_forcing_row_block(P.inflow, P.inflow_nrows, P.inflow_ncols, tim_new) : -1;
+    if (threadIdx.x == 0) clock_forcing(P, ctl, mrow, irow);
+}
+
+// Lane masks of the three ring columns (c-1, c, c+1, by ring slot), one bit
+// per lane of the wave, in scalar registers: w = wet in loop 1, a = an
+// interior cell that is not a weir cell (it computes fluxes).  A neighbour's
+// bit is the mask shifted by one lane (rows) or another slot's mask
+// (columns), so neighbour tests cost no vector instructions.
+struct cmask {
+    uint64_t w[3], a[3];
+};
+#define FLUX_LANES 0x7ffffffffffffffeull   // lanes 1..62 compute fluxes
 
 struct consts32 {
     float hdry, g, nuem, ks, dx, idx;
+    float sqrt_g, q_isg;               // sqrt(g), 1/(4 sqrt(g))
+    float nuem_idx;                    // nuem / dx
     float half_dx_over_dt, arbase_over_dt;
     double hdry64;
 };
@@ -169,53 +283,63 @@ __device__ inline void solver_wet32(const consts32 &K, const nb32 &n, const fron
         }
         lroe = false;
     }
-    const float cme = fx_sqrt(g * hme), cmn = fx_sqrt(g * hmn);
-    const float ihme = fx_rcp(fx_max_nn(hme, hdry)), ihmn = fx_rcp(fx_max_nn(hmn, hdry));
+    // one reciprocal square root per face, r = 1/sqrt(max(hm, hdry)), gives
+    // 1/max(hm, hdry) = r^2, the celerity sqrt(g hm) = sqrt(g) hm r and the
+    // Roe scale 1/(4c) = r/(4 sqrt(g)) where hm > hdry; where hm <= hdry the
+    // celerity stays finite and meets a zero scale
+    const float rse = fx_rsq(fx_max_nn(hme, hdry)), rsn = fx_rsq(fx_max_nn(hmn, hdry));
+    const float cme = K.sqrt_g * (hme * rse), cmn = K.sqrt_g * (hmn * rsn);
+    const float ihme = rse * rse, ihmn = rsn * rsn;
     const float ume = qme * ihme, vme = rme * ihme, umn = qmn * ihmn, vmn = rmn * ihmn;
     // eddy coefficients: us is identically zero, so cvdef*us*h + nuem = nuem;
     // a dry neighbour contributes h = 0 and velocity 0 (:124-126,151-153)
-    const float hne_t = K.nuem * n.sqP * (n.wetE ? n.sqE : 0.f);
-    const float hnn_t = K.nuem * n.sqP * (n.wetN ? n.sqN : 0.f);
+    const float hne_t = K.nuem_idx * n.sqP * (n.wetE ? n.sqE : 0.f);       // with the 1/dx of the gradients
+    const float hnn_t = K.nuem_idx * n.sqP * (n.wetN ? n.sqN : 0.f);
     const float un = n.wetN ? n.uN : 0.f;
     const float ve = n.wetE ? n.vE : 0.f;
     const float idx = K.idx;
-    const float txye = hne_t * ((ve - n.vP) * idx + .25f * (un + n.uNE - n.uS - n.uSE) * idx);
-    const float txyn = hnn_t * ((un - n.uP) * idx + .25f * (ve + n.vNE - n.vW - n.vNW) * idx);
+    const float txye = hne_t * ((ve - n.vP) + .25f * (un + n.uNE - n.uS - n.uSE));
+    const float txyn = hnn_t * ((un - n.uP) + .25f * (ve + n.vNE - n.vW - n.vNW));
     float fe1 = qme, fe2 = qme * ume, fe3 = qme * vme - txye;
     float fn1 = rmn, fn2 = rmn * umn - txyn, fn3 = rmn * vmn;
     if (lroe) {
-        float cc = 0.f;
+        // Roe dissipation -1/2 |A| dU (solver_wetdomain.cpp:218-277) in
+        // characteristic form: the strengths w1, w3 of the two gravity waves
+        // (speeds c3 = u - c and c2 = u + c, halved through cc = 1/(4c)) and of
+        // the shear wave (speed u, halved through hs) along the eigenvectors
+        // (1, c3, v), (1, c2, v) and (0, 0, 1).  It equals the reference's
+        // matrix form term by term, in about half the operations.
+        float cc = 0.f, hs = 0.f;
         if (hme > hdry) {
             float dzea = fabsf(dze);
             if (dzea > 0.5f * hme) dze = copysignf(fx_min_nn(dzea, fabsf(he - hp)), dze);
-            cc = .25f * fx_rcp(cme);
+            cc = K.q_isg * rse;
+            hs = .5f;
         }
         {
-            const float c1 = ume, c2 = ume + cme, c3 = ume - cme;
-            const float c1a = fabsf(c1), c2a = fabsf(c2), c3a = fabsf(c3);
-            const float a11 = c2 * c3a - c2a * c3, a12 = c2a - c3a;
-            const float a21 = c2 * c3 * (c3a - c2a), a22 = c2a * c2 - c3a * c3;
-            const float a31 = vme * (c2 * c3a - 2.f * cme * c1a - c2a * c3);
-            const float a32 = vme * (c2a - c3a), a33 = 2.f * cme * c1a;
-            fe1 -= (a11 * dze + a12 * dqe) * cc;
-            fe2 -= (a21 * dze + a22 * dqe) * cc;
-            fe3 -= (a31 * dze + a32 * dqe + a33 * dre) * cc;
+            const float c2 = ume + cme, c3 = ume - cme;
+            const float w1 = fabsf(c3) * ((c2 * dze - dqe) * cc);
+            const float w3 = fabsf(c2) * ((dqe - c3 * dze) * cc);
+            const float m1 = w1 + w3;
+            fe1 -= m1;
+            fe2 -= c3 * w1 + c2 * w3;
+            fe3 -= vme * m1 + hs * fabsf(ume) * (dre - vme * dze);
         }
         cc = 0.f;
+        hs = 0.f;
         if (hmn > hdry) {
             dzn = copysignf(fx_min_nn(fabsf(dzn), fabsf(hn - hp)), dzn);
-            cc = .25f * fx_rcp(cmn);
+            cc = K.q_isg * rsn;
+            hs = .5f;
         }
         {
-            const float c1 = vmn, c2 = vmn + cmn, c3 = vmn - cmn;
-            const float c1a = fabsf(c1), c2a = fabsf(c2), c3a = fabsf(c3);
-            const float a11 = c2 * c3a - c2a * c3, a13 = c2a - c3a;
-            const float a21 = umn * (c2 * c3a - 2.f * cmn * c1a - c2a * c3);
-            const float a22 = 2.f * cmn * c1a, a23 = umn * (c2a - c3a);
-            const float a31 = c2 * c3 * (c3a - c2a), a33 = c2a * c2 - c3a * c3;
-            fn1 -= (a11 * dzn + a13 * drn) * cc;
-            fn2 -= (a21 * dzn + a22 * dqn + a23 * drn) * cc;
-            fn3 -= (a31 * dzn + a33 * drn) * cc;
+            const float c2 = vmn + cmn, c3 = vmn - cmn;
+            const float w1 = fabsf(c3) * ((c2 * dzn - drn) * cc);
+            const float w3 = fabsf(c2) * ((drn - c3 * dzn) * cc);
+            const float m1 = w1 + w3;
+            fn1 -= m1;
+            fn2 -= umn * m1 + hs * fabsf(vmn) * (dqn - umn * dzn);
+            fn3 -= c3 * w1 + c2 * w3;
         }
     }
     fe2 += .5f * g * (hme * hme);
@@ -241,34 +365,23 @@ __device__ inline void solver_wet32(const consts32 &K, const nb32 &n, const fron
     f[0] = fe1; f[1] = fe2; f[2] = fe3; f[3] = fn1; f[4] = fn2; f[5] = fn3;
 }
 
-__device__ inline double wmin(double v)
-{
-    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ inline double wmax(double v)
-{
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ inline double wsum(double v)
-{
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-// CFL bound of one updated cell (main.cpp:308-316), FP64; h is the depth or
-// the signed d = z - zb (the same above hdry > 0)
-__device__ inline void cfl_cell(const fx_params &P, double h, float qx, float qy, double &dtm, double &hmx)
+// CFL bound of one updated cell (main.cpp:308-316).  The cell's bound is
+// cfl*dxy/(max(u0, v0) + c0), since min(a/(u0+c0), a/(v0+c0)) = a/(max(u0,v0)+c0)
+// (division is monotonic); the wave speeds max(u0, v0) + c0 are reduced by
+// their maximum, in FP32 (exact in any order, so the same for every tiling),
+// and the clock divides once (cfl_dt).  h is the depth or the signed
+// d = z - zb (the same above hdry > 0); the host holds hdry >= 1e-30, so hf is
+// a normal FP32 number.
+__device__ inline void cfl_cell(const fx_params &P, float sqrt_g, double h, float qx, float qy, float &smax,
+                                double &hmx)
 {
     if (h > P.hdry) {
         hmx = fmax(hmx, h);
-        // min(a/(u0+c0), a/(v0+c0)) = a/(max(u0,v0)+c0) (division is monotonic);
-        // the bound is evaluated in FP32, the reduction over cells in FP64
         const float hf = (float)h;
-        const float c0 = fx_sqrt((float)P.g * hf);
-        const float u = fx_max_nn(1e-6f, fx_max_nn(fabsf(qx), fabsf(qy)) * fx_rcp(hf));
-        dtm = fmin(dtm, (double)(P.cfl_dxy * fx_rcp(u + c0)));
+        const float r = fx_rsq(hf);                          // 1/h = r^2, sqrt(g h) = sqrt(g) h r
+        const float c0 = sqrt_g * (hf * r);
+        const float u = fx_max_nn(1e-6f, fx_max_nn(fabsf(qx), fabsf(qy)) * (r * r));
+        smax = fx_max_nn(smax, u + c0);
     }
 }
 
@@ -297,19 +410,21 @@ struct pref {                 // one cell's HBM values in flight
 #define PF_INFLOW 0x100u      // the cell takes the inflow point source
 
 // Three columns (c-1, c, c+1) of what the flux stencil reads from the S
-// neighbour (u and the wet flags); columns c and c+1 of the rest, and the
+// neighbour (u; the wet flags are lane masks, cmask); columns c and c+1 of the rest, and the
 // fluxes only where the update reads them: the E face of column c and the N
-// faces of columns c-1 and c.  The two-column arrays are indexed by column
-// parity.  8000 bytes per wave.
+// faces of columns c-1 and c.  What two columns keep sits in one record per
+// column parity.  7808 bytes per wave.
+struct ring2 {
+    float dm[NS], qx[NS], qy[NS], v[NS], sq[NS];
+    double zb[NS];            // FP64 bed
+    double dm64[NS];          // FP64 depth after melt
+    float fn[3][NS];          // fn_1..fn_3 of this column
+    uint8_t raw[NS];          // flag bytes
+};
 struct ring {
     float u[3][NS];
-    float dm[2][NS], qx[2][NS], qy[2][NS], v[2][NS], sq[2][NS];
-    double zb[2][NS];          // FP64 bed
-    uint8_t fl[3][NS];        // LW | LX | LI | LS
-    uint8_t raw[2][NS];       // flag bytes
-    double dm64[2][NS];       // FP64 depth after melt
     float fe[3][NS];          // fe_1..fe_3 of column c
-    float fn[2][3][NS];       // fn_1..fn_3 of columns c-1 and c
+    ring2 p[2];               // by column parity
 };
 
 // the FP64 front of row i (columns of parity pc and pn = pc ^ 1): depths
@@ -317,18 +432,18 @@ struct ring {
 __device__ inline front64 front_of(const ring &R, int pc, int pn, int i)
 {
     front64 fr;
-    fr.dp = R.dm64[pc][i];
-    fr.de = R.dm64[pc][i + 1];
-    fr.dn = R.dm64[pn][i];
-    fr.dzbE = R.zb[pc][i + 1] - R.zb[pc][i];
-    fr.dzbN = R.zb[pn][i] - R.zb[pc][i];
+    fr.dp = R.p[pc].dm64[i];
+    fr.de = R.p[pc].dm64[i + 1];
+    fr.dn = R.p[pn].dm64[i];
+    fr.dzbE = R.p[pc].zb[i + 1] - R.p[pc].zb[i];
+    fr.dzbN = R.p[pn].zb[i] - R.p[pc].zb[i];
     return fr;
 }
 
 // flux m (0..5 = fe_1..fe_3, fn_1..fn_3) of row i of the column of parity par
 __device__ inline float &rflux(ring &R, int par, int m, int i)
 {
-    return m < 3 ? R.fe[m][i] : R.fn[par][m - 3][i];
+    return m < 3 ? R.fe[m][i] : R.p[par].fn[m - 3][i];
 }
 
 // per-block constants of the sweep
@@ -342,9 +457,9 @@ struct sweep_ctx {
     consts32 K;
     double meteoi, inflowi, s_wet_prev;
     uint32_t inflow_p;        // linear index of the inflow cell, ~0u = none
-    float dxy_over_dt, kf, dtf, hdryf;
+    float dxy_over_dt, kf, dtf, dtf1, hdryf;     // dtf1 = dt / 10
     int gate, ade_on, k, row, mcols;
-    bool row_ok, flux_lane, own;
+    bool row_ok, own;
     uint32_t ld;
 };
 
@@ -365,12 +480,12 @@ __device__ inline void load_col(const sweep_ctx &X, int c, pref &a)
     if (p == X.inflow_p) a.f |= PF_INFLOW;
 }
 
-// L: this lane's cell of the column into ring slot SL (melt of this step,
-// hydro loop 1, cell velocities)
+// L: this lane's cell of the column into ring slots SL (three-column arrays)
+// and par (two-column record): melt of this step, hydro loop 1, cell velocities
 template <int SL>
-__device__ inline void store_col(const sweep_ctx &X, ring &R, const pref &a, int col, bool own_col, double &ms)
+__device__ inline void store_col(const sweep_ctx &X, ring &R, cmask &M, const pref &a, int par, bool own_col,
+                                 double &ms)
 {
-    const int par = col & 1;
     const int i = X.k;
     const bool inner = a.f & FXF_INNER;
     // melt area that brings solute (cells holding water before the melt,
@@ -388,33 +503,32 @@ __device__ inline void store_col(const sweep_ctx &X, ring &R, const pref &a, int
     const float dm = (float)d;
     const float h = fx_max0(dm);
     const float den = fx_rcp(fx_max_nn(fx_max_nn(h, X.hdryf), X.K.ks));
-    R.dm[par][i] = dm;
-    R.qx[par][i] = qx;
-    R.qy[par][i] = qy;
+    R.p[par].dm[i] = dm;
+    R.p[par].qx[i] = qx;
+    R.p[par].qy[i] = qy;
     R.u[SL][i] = qx * den;
-    R.v[par][i] = qy * den;
-    R.sq[par][i] = fx_sqrt(h);
-    R.zb[par][i] = inner ? a.zb : 0.0;
-    R.fl[SL][i] = (uint8_t)((wet ? LW : 0) | ((a.f & FXF_WEIR) ? LX : 0) | (inner ? LI : 0) |
-                            ((a.f & FXF_STALE) ? LS : 0));
-    R.raw[par][i] = (uint8_t)a.f;
-    R.dm64[par][i] = d;
+    R.p[par].v[i] = qy * den;
+    R.p[par].sq[i] = fx_sqrt(h);
+    R.p[par].zb[i] = inner ? a.zb : 0.0;
+    M.w[SL] = __builtin_amdgcn_ballot_w64(wet);
+    M.a[SL] = __builtin_amdgcn_ballot_w64(inner && !(a.f & FXF_WEIR));
+    R.p[par].raw[i] = (uint8_t)a.f;
+    R.p[par].dm64[i] = d;
 }
 
-// F(c): fluxes of this lane's row; column c in slot S0, c-1 in SS, c+1 in SN
+// F(c): fluxes of this lane's row; column c in slots S0 (three-column arrays)
+// and pc = c & 1 (two-column record), c-1 in SS, c+1 in SN and pc ^ 1
 template <int S0, int SS, int SN>
-__device__ inline void flux_col(const sweep_ctx &X, ring &R, int c)
+__device__ inline void flux_col(const sweep_ctx &X, ring &R, const cmask &M, int c)
 {
     const int i = X.k;
     float f[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    const uint8_t fl = R.fl[S0][i];
-    if (X.gate && X.flux_lane && (fl & LI) && !(fl & LX)) {
-        const bool wetE = R.fl[S0][i + 1] & LW, wetN = R.fl[SN][i] & LW;
+    if (X.gate && lane_bit(M.a[S0] & FLUX_LANES)) {
+        const bool wetE = lane_bit(M.w[S0] >> 1), wetN = lane_bit(M.w[SN]);
         const int pc = c & 1, pn = pc ^ 1;     // slots of columns c and c+1
-        if (!(fl & LW)) {
-            const bool wetW = R.fl[S0][i - 1] & LW, wetS = R.fl[SS][i] & LW;
-            if (!wetW && !wetE && !wetS && !wetN) {
-                if (fl & LS) {                     // stale fluxes of an all-dry neighbourhood
+        if (!lane_bit(M.w[S0])) {
+            if (!lane_bit((M.w[S0] << 1) | (M.w[S0] >> 1) | M.w[SS] | M.w[SN])) {
+                if (R.p[pc].raw[i] & FXF_STALE) {  // stale fluxes of an all-dry neighbourhood
                     const uint32_t p = (uint32_t)X.row + (uint32_t)c * X.ld;
                     const float2 *s6 = reinterpret_cast<const float2 *>(X.S->f6 + (size_t)p * 6);
                     const float2 a = s6[0], b = s6[1], e = s6[2];
@@ -424,21 +538,21 @@ __device__ inline void flux_col(const sweep_ctx &X, ring &R, int c)
                 solver_dry32(X.K, front_of(R, pc, pn, i), wetE, wetN, X.dxy_over_dt, f);
             }
         } else {
-            const float dzbE = (float)(R.zb[pc][i + 1] - R.zb[pc][i]);
-            const float dzbN = (float)(R.zb[pn][i] - R.zb[pc][i]);
+            const float dzbE = (float)(R.p[pc].zb[i + 1] - R.p[pc].zb[i]);
+            const float dzbN = (float)(R.p[pn].zb[i] - R.p[pc].zb[i]);
             front64 fr;
             if (!wetE || !wetN) fr = front_of(R, pc, pn, i);
             nb32 n;
-            n.dp = R.dm[pc][i]; n.de = R.dm[pc][i + 1]; n.dn = R.dm[pn][i];
+            n.dp = R.p[pc].dm[i]; n.de = R.p[pc].dm[i + 1]; n.dn = R.p[pn].dm[i];
             n.dzbE = dzbE; n.dzbN = dzbN;
-            n.qp = R.qx[pc][i]; n.qe = R.qx[pc][i + 1]; n.qn = R.qx[pn][i];
-            n.rp = R.qy[pc][i]; n.re = R.qy[pc][i + 1]; n.rn = R.qy[pn][i];
-            n.uP = R.u[S0][i]; n.vP = R.v[pc][i];
+            n.qp = R.p[pc].qx[i]; n.qe = R.p[pc].qx[i + 1]; n.qn = R.p[pn].qx[i];
+            n.rp = R.p[pc].qy[i]; n.re = R.p[pc].qy[i + 1]; n.rn = R.p[pn].qy[i];
+            n.uP = R.u[S0][i]; n.vP = R.p[pc].v[i];
             n.uN = R.u[SN][i]; n.uS = R.u[SS][i];
             n.uSE = R.u[SS][i + 1]; n.uNE = R.u[SN][i + 1];
-            n.vE = R.v[pc][i + 1]; n.vW = R.v[pc][i - 1];
-            n.vNW = R.v[pn][i - 1]; n.vNE = R.v[pn][i + 1];
-            n.sqP = R.sq[pc][i]; n.sqE = R.sq[pc][i + 1]; n.sqN = R.sq[pn][i];
+            n.vE = R.p[pc].v[i + 1]; n.vW = R.p[pc].v[i - 1];
+            n.vNW = R.p[pn].v[i - 1]; n.vNE = R.p[pn].v[i + 1];
+            n.sqP = R.p[pc].sq[i]; n.sqE = R.p[pc].sq[i + 1]; n.sqN = R.p[pn].sq[i];
             n.wetE = wetE; n.wetN = wetN;
             solver_wet32(X.K, n, fr, f);
         }
@@ -447,27 +561,26 @@ __device__ inline void flux_col(const sweep_ctx &X, ring &R, int c)
     for (int m = 0; m < 6; m++) rflux(R, fs, m, i) = f[m];
 }
 
-// U(c): hydro loops 3 and 4 for this lane's own cell, and its CFL bound
+// U(c): hydro loops 3 and 4 for this lane's own cell, and its wave speed
 template <int S0, int SS, int SN>
-__device__ inline void update_col(const sweep_ctx &X, ring &R, int c, double &dtm, double &hmx,
+__device__ inline void update_col(const sweep_ctx &X, ring &R, const cmask &M, int c, float &smax, double &hmx,
                                   double &wv, bool &bad)
 {
     if (!X.own) return;
     const fx_params &P = *X.P;
     const s32p &S = *X.S;
-    const int i = X.k, fs = c & 1, fsS = (c - 1) & 1;
+    const int i = X.k, fs = c & 1, fsS = fs ^ 1;
     const uint32_t p = (uint32_t)X.row + (uint32_t)c * X.ld;
     const uint32_t o4 = p * 4u;
-    const uint8_t f = R.raw[c & 1][i];
-    const uint8_t fl = R.fl[S0][i];
-    const bool weir = f & FXF_WEIR;
-    double d = R.dm64[c & 1][i];
+    const uint8_t f = R.p[fs].raw[i];
+    const bool weir = !lane_bit(M.a[S0]);             // own cells are interior: weir = no fluxes
+    double d = R.p[fs].dm64[i];
     float qxn, qyn;
     uint8_t nf = f;
     float fe1P = 0.f, fn1P = 0.f;
     if (!X.gate) {
-        qxn = R.qx[c & 1][i];
-        qyn = R.qy[c & 1][i];
+        qxn = R.p[fs].qx[i];
+        qyn = R.p[fs].qy[i];
         // no solver step, so every cell was at or below hdry at the CFL block,
         // which marks it dry (main.cpp:300-325); add_meteo, when there is a
         // melt table, then marks wet each cell its melt leaves at or below
@@ -488,8 +601,11 @@ __device__ inline void update_col(const sweep_ctx &X, ring &R, int c, double &dt
             d += ((double)a1 - (double)b1) + ((double)c1 - (double)e1);
             dqx = ((rflux(R, fs, 1, i - 1) - rflux(R, fs, 1, i)) + (rflux(R, fsS, 4, i) - rflux(R, fs, 4, i))) * kf;
             dqy = ((rflux(R, fs, 2, i - 1) - rflux(R, fs, 2, i)) + (rflux(R, fsS, 5, i) - rflux(R, fs, 5, i))) * kf;
-            if (R.fl[S0][i + 1] & LX) wv -= (double)b1;        // into / out of a weir cell
-            if (R.fl[SN][i] & LX) wv -= (double)e1;
+            // into / out of a weir cell: the neighbours of a non-weir cell
+            // are interior cells (the outer interior ring is weir), so a
+            // neighbour that computes no fluxes is a weir cell
+            if (!lane_bit(M.a[S0] >> 1)) wv -= (double)b1;
+            if (!lane_bit(M.a[SN])) wv -= (double)e1;
         }
         // h = max(0, d) against hdry > 0 is d against hdry
         // the reference's loop-4 wet flag (hydrodynamics_calc.cpp:123-154)
@@ -499,14 +615,16 @@ __device__ inline void update_col(const sweep_ctx &X, ring &R, int c, double &dt
             qxn = 0.f;
             qyn = 0.f;
         } else {
-            qxn = .1f * (fe1W * dtf) + .8f * (R.qx[c & 1][i] + dqx) + .1f * (fe1P * dtf);
-            qyn = .1f * (fn1S * dtf) + .8f * (R.qy[c & 1][i] + dqy) + .1f * (fn1P * dtf);
+            // 0.1 of each face's discharge and 0.8 of the updated cell's
+            // (hydrodynamics_calc.cpp:145-150), the two faces summed first
+            qxn = (fe1W + fe1P) * X.dtf1 + .8f * (R.p[fs].qx[i] + dqx);
+            qyn = (fn1S + fn1P) * X.dtf1 + .8f * (R.p[fs].qy[i] + dqy);
         }
         // keep the fluxes of a cell that may be all-dry next step (S1)
-        const bool wet1 = fl & LW;
+        const bool wet1 = lane_bit(M.w[S0]);
         if (!weir && (d <= P.hdry || !P.rates_nonneg)) {
-            const bool alldry = !wet1 && !(R.fl[S0][i - 1] & LW) && !(R.fl[S0][i + 1] & LW) &&
-                                !(R.fl[SS][i] & LW) && !(R.fl[SN][i] & LW);
+            const bool alldry =
+                !lane_bit(M.w[S0] | (M.w[S0] << 1) | (M.w[S0] >> 1) | M.w[SS] | M.w[SN]);
             if (!alldry) {
                 bool nz = false;
                 for (int m = 0; m < 6; m++) nz |= rflux(R, fs, m, i) != 0.f;
@@ -545,22 +663,29 @@ __device__ inline void update_col(const sweep_ctx &X, ring &R, int c, double &dt
         at_off(S.ff, p * 8u) = make_float2(fe1P, fn1P);
     }
     if (nf != f) at_off(S.flags, p) = nf;
-    cfl_cell(P, d, qxn, qyn, dtm, hmx);
+    cfl_cell(P, X.K.sqrt_g, d, qxn, qyn, smax, hmx);
 }
 
-// one column of the sweep; SC = ring slot of column c
+// one column of the sweep; SC = ring slot of column c in the three-column
+// arrays (column c0 + j uses j % 3); its two-column record is c & 1
 template <int SC>
-__device__ inline void column_step(const sweep_ctx &X, ring &R, int c, int c_last, pref &a,
-                                   double &dtm, double &hmx, double &wv, double &ms, bool &bad)
+__device__ inline void column_step(const sweep_ctx &X, ring &R, cmask &M, int c, int c_last, pref &a,
+                                   float &smax, double &hmx, double &wv, double &ms, bool &bad)
 {
     constexpr int SN = (SC + 1) % 3, SS = (SC + 2) % 3;
     __syncthreads();
-    store_col<SN>(X, R, a, c + 1, c + 1 <= c_last, ms);   // column c+1 (prefetched last step)
+    store_col<SN>(X, R, M, a, (c + 1) & 1, c + 1 <= c_last, ms);   // column c+1 (prefetched last step)
     if (c + 2 <= c_last + 1) load_col(X, c + 2, a);
     __syncthreads();
-    flux_col<SC, SS, SN>(X, R, c);
+    flux_col<SC, SS, SN>(X, R, M, c);
     __syncthreads();
-    update_col<SC, SS, SN>(X, R, c, dtm, hmx, wv, bad);
+    update_col<SC, SS, SN>(X, R, M, c, smax, hmx, wv, bad);
+}
+
+__device__ inline float wmaxf_nn(float v)            // v >= 0
+{
+    for (int o = 32; o > 0; o >>= 1) v = fx_max_nn(v, __shfl_xor(v, o, 64));
+    return v;
 }
 
 // one block of the sweep: strip bx (rows), column block by; gx = strips per
@@ -577,11 +702,11 @@ __device__ inline void sweep_tile(const fx_params &P, const s32p &S, fx_ctl *ctl
     X.d_out = const_cast<double *>(S.d_in[cur ^ 1]);
     X.q_in = S.q_in[cur];
     X.q_out = const_cast<float2 *>(S.q_in[cur ^ 1]);
-    X.gate = ctl->gate;
-    X.ade_on = ade_on;
     X.meteoi = P.meteo ? ctl->meteoi : 0.0;
     X.inflowi = ctl->inflowi;
     X.inflow_p = P.inflow ? (uint32_t)P.inflow_p : ~0u;
+    X.gate = ctl->gate;
+    X.ade_on = ade_on;
     X.s_wet_prev = ctl->s_wet_prev;
     X.k = threadIdx.x;
     const int r0 = 1 + bx * TH;
@@ -589,7 +714,6 @@ __device__ inline void sweep_tile(const fx_params &P, const s32p &S, fx_ctl *ctl
     const int c_last = min(c0 + P.ch - 1, P.ncols);
     X.row = r0 - 2 + X.k;                             // this lane's row
     X.row_ok = X.row >= 0 && X.row < P.ld;
-    X.flux_lane = X.k >= 1 && X.k <= NS - 2;
     X.own = X.k >= 2 && X.k <= NS - 2 && X.row <= P.nrows;
     X.mcols = P.ncols + 2;
     X.ld = (uint32_t)P.ld;
@@ -597,43 +721,49 @@ __device__ inline void sweep_tile(const fx_params &P, const s32p &S, fx_ctl *ctl
     X.K.hdry = (float)P.hdry;
     X.K.hdry64 = P.hdry;
     X.K.g = (float)P.g;
+    X.K.sqrt_g = fx_sqrt(X.K.g);
+    X.K.q_isg = .25f * fx_rcp(X.K.sqrt_g);
     X.K.nuem = (float)P.nuem;
     X.K.ks = (float)P.ks0;
     X.K.dx = (float)P.dxy;
     X.K.idx = 1.f / X.K.dx;
+    X.K.nuem_idx = X.K.nuem * X.K.idx;
     X.K.half_dx_over_dt = ctl->k_half_dx_dt;
     X.K.arbase_over_dt = ctl->k_arbase_dt;
     X.dxy_over_dt = ctl->k_dxy_dt;
     X.kf = ctl->k_dt_dxy;
     X.dtf = ctl->k_dt;
+    X.dtf1 = .1f * X.dtf;
 
     pref a;
-    double dtm = 9.e10, hmx = 0.0, wv = 0.0, ms = 0.0;
+    float smax = 0.f;
+    double hmx = 0.0, wv = 0.0, ms = 0.0;
     bool bad = false;
     // prologue: columns c0-2 (slot 1), c0-1 (slot 2), c0 (slot 0), fluxes of c0-1
-    load_col(X, c0 - 2, a); store_col<1>(X, R, a, c0 - 2, false, ms);
-    load_col(X, c0 - 1, a); store_col<2>(X, R, a, c0 - 1, false, ms);
-    load_col(X, c0, a); store_col<0>(X, R, a, c0, true, ms);
+    cmask M;
+    load_col(X, c0 - 2, a); store_col<1>(X, R, M, a, (c0 - 2) & 1, false, ms);
+    load_col(X, c0 - 1, a); store_col<2>(X, R, M, a, (c0 - 1) & 1, false, ms);
+    load_col(X, c0, a); store_col<0>(X, R, M, a, c0 & 1, true, ms);
     load_col(X, c0 + 1, a);
     __syncthreads();
-    flux_col<2, 1, 0>(X, R, c0 - 1);
+    flux_col<2, 1, 0>(X, R, M, c0 - 1);
     // column c uses ring slot (c - c0) % 3, unrolled by three
     int c = c0;
     for (; c + 2 <= c_last; c += 3) {
-        column_step<0>(X, R, c, c_last, a, dtm, hmx, wv, ms, bad);
-        column_step<1>(X, R, c + 1, c_last, a, dtm, hmx, wv, ms, bad);
-        column_step<2>(X, R, c + 2, c_last, a, dtm, hmx, wv, ms, bad);
+        column_step<0>(X, R, M, c, c_last, a, smax, hmx, wv, ms, bad);
+        column_step<1>(X, R, M, c + 1, c_last, a, smax, hmx, wv, ms, bad);
+        column_step<2>(X, R, M, c + 2, c_last, a, smax, hmx, wv, ms, bad);
     }
-    if (c <= c_last) { column_step<0>(X, R, c, c_last, a, dtm, hmx, wv, ms, bad); c++; }
-    if (c <= c_last) { column_step<1>(X, R, c, c_last, a, dtm, hmx, wv, ms, bad); c++; }
-    dtm = wmin(dtm);
+    if (c <= c_last) { column_step<0>(X, R, M, c, c_last, a, smax, hmx, wv, ms, bad); c++; }
+    if (c <= c_last) { column_step<1>(X, R, M, c, c_last, a, smax, hmx, wv, ms, bad); c++; }
+    smax = wmaxf_nn(smax);
     hmx = wmax(hmx);
     wv = wsum(wv);
     ms = wsum(ms);
     if (__any(bad) && X.k == 0) atomicOr(&ctl->status, 4);
     if (X.k == 0) {
         const size_t bid = bx + (size_t)by * gx;
-        S.partials[FX32_NPV * bid] = dtm;
+        S.partials[FX32_NPV * bid] = (double)smax;
         S.partials[FX32_NPV * bid + 1] = hmx;
         S.partials[FX32_NPV * bid + 2] = wv;
         S.partials[FX32_NPV * bid + 3] = ms;
@@ -649,7 +779,7 @@ __global__ __launch_bounds__(64) void k32_sweep(fx_params P, s32p S, fx_ctl *ctl
 }
 
 // batched: block b runs block FX_BMAP_BLK(tmap[b]) of member FX_BMAP_SLOT(tmap[b])
-__global__ __launch_bounds__(64) void k32b_sweep(const fx_bslot *slots, const uint32_t *tmap)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k32b_sweep(const fx_bslot *slots, const uint32_t *tmap)
 {
     __shared__ ring R;
     const uint32_t m = tmap[blockIdx.x];
@@ -668,18 +798,20 @@ __global__ __launch_bounds__(64) void k32_cfl(fx_params P, s32p S, fx_ctl *ctl)
     const int k = threadIdx.x;
     const int r0 = 1 + blockIdx.x * TH, c0 = 1 + blockIdx.y * P.ch;
     const int row = r0 - 2 + k;
-    double dtm = 9.e10, hmx = 0.0;
+    const float sqrt_g = fx_sqrt((float)P.g);
+    float smax = 0.f;
+    double hmx = 0.0;
     if (k >= 2 && k <= NS - 2 && row <= P.nrows) {
         for (int c = c0; c <= min(c0 + P.ch - 1, P.ncols); c++) {
             const uint32_t p = (uint32_t)row + (uint32_t)c * (uint32_t)P.ld;
-            cfl_cell(P, d_in[p], q_in[p].x, q_in[p].y, dtm, hmx);
+            cfl_cell(P, sqrt_g, d_in[p], q_in[p].x, q_in[p].y, smax, hmx);
         }
     }
-    dtm = wmin(dtm);
+    smax = wmaxf_nn(smax);
     hmx = wmax(hmx);
     if (k == 0) {
         const size_t bid = blockIdx.x + (size_t)blockIdx.y * gridDim.x;
-        S.partials[FX32_NPV * bid] = dtm;
+        S.partials[FX32_NPV * bid] = (double)smax;
         S.partials[FX32_NPV * bid + 1] = hmx;
     }
 }
@@ -688,15 +820,15 @@ __global__ __launch_bounds__(64) void k32_cfl(fx_params P, s32p S, fx_ctl *ctl)
 // partials left by the previous step, clip to the print time, advance tim,
 // evaluate the forcing, gate the solver, update the FP64 water ledger.
 //
-// clock_fold: one block's min dt / max hpall / weir-volume sum over partials
+// clock_fold: one block's max wave speed / max hpall / weir-volume sum over partials
 // [lo, hi), in a fixed order (the weir entries are consumed: set to 0).
 __device__ inline void clock_fold(double *partials, int lo, int hi, double &a, double &b, double &c, double &m)
 {
     const int tid = threadIdx.x;
-    a = 9.e10; b = 0.0; c = 0.0; m = 0.0;
+    a = 0.0; b = 0.0; c = 0.0; m = 0.0;
     for (int i = lo + tid; i < hi; i += blockDim.x) {
         double *q = partials + FX32_NPV * (size_t)i;
-        a = fmin(a, q[0]);
+        a = fmax(a, q[0]);
         b = fmax(b, q[1]);
         c += q[2];
         m += q[3];
@@ -704,97 +836,13 @@ __device__ inline void clock_fold(double *partials, int lo, int hi, double &a, d
         q[3] = 0.0;
     }
     __shared__ double sm[4][16];
-    a = wmin(a); b = wmax(b); c = wsum(c); m = wsum(m);
+    a = wmax(a); b = wmax(b); c = wsum(c); m = wsum(m);
     if ((tid & 63) == 0) { sm[0][tid >> 6] = a; sm[1][tid >> 6] = b; sm[2][tid >> 6] = c; sm[3][tid >> 6] = m; }
     __syncthreads();
     if (tid == 0)
         for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
-            a = fmin(a, sm[0][w]); b = fmax(b, sm[1][w]); c += sm[2][w]; m += sm[3][w];
+            a = fmax(a, sm[0][w]); b = fmax(b, sm[1][w]); c += sm[2][w]; m += sm[3][w];
         }
-}
-
-__device__ inline void clock_advance(const fx_params &P, fx_ctl *ctl, double a, double b, double c, double m);
-__device__ inline void clock_forcing(const fx_params &P, fx_ctl *ctl, int mrow, int irow);
-
-// clock_finish: the step's clock from the folded (dt, hpall, weir).  Every
-// thread of the block calls it (the forcing-table lookup is shared); thread 0
-// holds the folded values and writes the control block.
-__device__ inline void clock_finish(const fx_params &P, fx_ctl *ctl, double a, double b, double c, double m)
-{
-    __shared__ double tim_new;
-    if (threadIdx.x == 0) clock_advance(P, ctl, a, b, c, m);
-    if (threadIdx.x == 0) tim_new = ctl->tim;
-    __syncthreads();
-    const int mrow = P.meteo ? fx_forcing_row_block(P.meteo, P.meteo_nrows, P.meteo_ncols, tim_new) : -1;
-    const int irow = P.inflow ? fx_forcing_row_block(P.inflow, P.inflow_nrows, P.inflow_ncols, tim_new) : -1;
-    if (threadIdx.x == 0) clock_forcing(P, ctl, mrow, irow);
-}
-
-__device__ inline void kahan_add(double &sum, double &comp, double v)
-{
-    const double y = v - comp;
-    const double t = sum + y;
-    comp = (t - sum) - y;
-    sum = t;
-}
-
-__device__ inline void clock_advance(const fx_params &P, fx_ctl *ctl, double a, double b, double c, double m)
-{
-    // Kahan-compensated ledger of the weir-face exchange of the previous step
-    kahan_add(ctl->led_weir, ctl->led_weir_c, c * P.arbase);
-    // solute the previous step's melt brought into water (m: melt area of
-    // the cells that held water; meteoi and cmelt are still that step's)
-    if (P.meteo && P.ade)
-        for (int k = 0; k < P.nchem; k++)
-            kahan_add(ctl->sled[k], ctl->sled_c[k], ctl->meteoi * m * P.arbase * ctl->cmelt[k]);
-    ctl->dt_raw = a;
-    const double dt = fmin(ctl->print_next - ctl->tim, a);
-    ctl->dtfl = dt;
-    ctl->hpall = b;
-    ctl->tim = ctl->tim + dt;
-    const int gate = (b != 0.0);
-    ctl->gate = gate;
-    ctl->it += gate;
-    ctl->steps += 1;
-    ctl->solver_steps += gate;
-    ctl->k_dt_dxy = (float)(dt / P.dxy);
-    ctl->k_dt = (float)dt;
-    ctl->k_half_dx_dt = (float)(0.5 * P.dxy / dt);
-    ctl->k_arbase_dt = (float)(P.arbase / dt);
-    ctl->k_dxy_dt = (float)(P.dxy / dt);
-    ctl->s_wet_prev = ctl->s_wet;
-    if (gate) ctl->s_wet = ctl->s_wet + dt / 3600;
-}
-
-__device__ inline void clock_forcing(const fx_params &P, fx_ctl *ctl, int mrow, int irow)
-{
-    fx_forcing_eval(P, ctl, mrow, irow);
-    if (P.inflow) {                  // the inflow cell's bed is never NODATA here (checked by the host)
-        // water in: the source, plus a deficit of the cell filled up to the
-        // bed first (common.cpp:324-325), judged after this step's melt
-        double vin = ctl->inflowi;
-        if (P.d32[0]) {
-            const size_t p = (size_t)P.inflow_p;
-            double dm = P.d32[ctl->cur][p];
-            if (P.meteo) dm += ctl->meteoi * (P.melt_mult ? (double)P.melt_mult[p] : 1.0);
-            if (dm <= 0.0) vin -= dm;
-            // its solute enters only water (common.cpp:350-361)
-            if (dm > 0.0 && P.ade)
-                for (int k = 0; k < P.nchem; k++)
-                    kahan_add(ctl->sled[k], ctl->sled_c[k], ctl->inflowi * P.arbase * ctl->cin[k]);
-        }
-        const double y = vin * P.arbase - ctl->led_melt_c;
-        const double t = ctl->led_melt + y;
-        ctl->led_melt_c = (t - ctl->led_melt) - y;
-        ctl->led_melt = t;
-    }
-    if (P.meteo) {
-        const double y = ctl->meteoi * ctl->melt_area * P.arbase - ctl->led_melt_c;
-        const double t = ctl->led_melt + y;
-        ctl->led_melt_c = (t - ctl->led_melt) - y;
-        ctl->led_melt = t;
-    }
-    ctl->pending_halt = (ctl->tim >= ctl->print_next);
 }
 
 // single block (batched members)
@@ -839,22 +887,22 @@ __global__ __launch_bounds__(1024) void k32_clock(fx_params P, double *partials,
     // second level, by the whole block: the other blocks' part2 entries are
     // read past this CU's cache (volatile), one entry per thread
     __threadfence();
-    double A = 9.e10, B = 0.0, Cw = 0.0, Mw = 0.0;
+    double A = 0.0, B = 0.0, Cw = 0.0, Mw = 0.0;
     for (int k = threadIdx.x; k < (int)gridDim.x; k += blockDim.x) {
         const volatile double *q = part2 + FX32_NPV * (size_t)k;
-        A = fmin(A, q[0]);
+        A = fmax(A, q[0]);
         B = fmax(B, q[1]);
         Cw += q[2];
         Mw += q[3];
     }
     __shared__ double sm2[4][16];
-    A = wmin(A); B = wmax(B); Cw = wsum(Cw); Mw = wsum(Mw);
+    A = wmax(A); B = wmax(B); Cw = wsum(Cw); Mw = wsum(Mw);
     const int tid = threadIdx.x;
     if ((tid & 63) == 0) { sm2[0][tid >> 6] = A; sm2[1][tid >> 6] = B; sm2[2][tid >> 6] = Cw; sm2[3][tid >> 6] = Mw; }
     __syncthreads();
     if (tid == 0) {
         for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
-            A = fmin(A, sm2[0][w]); B = fmax(B, sm2[1][w]); Cw += sm2[2][w]; Mw += sm2[3][w];
+            A = fmax(A, sm2[0][w]); B = fmax(B, sm2[1][w]); Cw += sm2[2][w]; Mw += sm2[3][w];
         }
         *ticket = 0;
     }

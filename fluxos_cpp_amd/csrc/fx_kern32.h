@@ -60,13 +60,18 @@ __device__ inline bool fx_finitef(float v)
 // two floats of which at least one is non-negative (>= +0), compare the bit
 // patterns as signed integers: for such pairs the integer order is the float
 // order, and no NaN quieting of the operands is needed as for fmaxf / fminf.
-// The reciprocal and square root are the hardware's (1 ulp; FP32 denormals
-// are flushed in these kernels anyway).
+// The reciprocal, square root and reciprocal square root are the hardware's
+// (1 ulp; FP32 denormals are flushed in these kernels anyway).
 __device__ inline float fx_max0(float x) { return __int_as_float(max(0, __float_as_int(x))); }
 __device__ inline float fx_max_nn(float a, float b) { return __int_as_float(max(__float_as_int(a), __float_as_int(b))); }
 __device__ inline float fx_min_nn(float a, float b) { return __int_as_float(min(__float_as_int(a), __float_as_int(b))); }
 __device__ inline float fx_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ inline float fx_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ inline float fx_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// a uniform 64-bit lane mask as this lane's flag: no instruction, the mask is
+// used as the lane predicate (the inverse of a ballot)
+__device__ inline bool lane_bit(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
 
 #define FX32_TH 61            // rows owned per step-sweep strip
 #define FX32_CH 32            // default columns per step-sweep block
@@ -74,7 +79,7 @@ __device__ inline float fx_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 #define FX32_AOWN 58          // rows owned per ADE-sweep strip
 #define FX32_ACH 64           // default columns per ADE-sweep block
 #define FX32_FOLD_MAX 256     // blocks of the clock's partial fold
-#define FX32_NPV 4            // values per step-sweep partial: min dt, max h, weir volume, solute melt area
+#define FX32_NPV 4            // values per step-sweep partial: max wave speed, max h, weir volume, solute melt area
 
 // One member of a batch, in device memory: everything its kernels need.
 struct fx_bslot {
