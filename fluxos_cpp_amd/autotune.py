@@ -15,7 +15,9 @@ keeps the fastest for each kernel; ``main`` tunes a ladder of sizes and writes
 ``ch``: blocks run in rounds of the device's wave slots, and a last round
 that is nearly empty costs as much as a full one.
 
-    python -m fluxos_cpp_amd.autotune [--sizes 512,1024,2048,4096,8192,16384] [--steps 40]
+    python -m fluxos_cpp_amd.autotune [--sizes 512,1024,2048,4096,8192,16384] [--steps 40] [--merge]
+
+``--merge`` replaces only the tuned sizes' entries of an existing table.
 """
 from __future__ import annotations
 
@@ -28,7 +30,7 @@ from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 TABLE = PKG / "tuning.json"
-CANDIDATES_CH = (8, 12, 16, 20, 24, 28, 32, 40, 48, 64)
+CANDIDATES_CH = (8, 12, 16, 20, 22, 23, 24, 26, 28, 32, 40, 48, 64)
 CANDIDATES_ACH = (16, 32, 48, 64, 96, 128)
 FX32_ACH_DEFAULT = 64
 
@@ -147,6 +149,7 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--out", default=str(TABLE))
     ap.add_argument("--batch", action="store_true", help="also tune the batch entry (about a minute per candidate)")
+    ap.add_argument("--merge", action="store_true", help="keep the table's entries of the sizes not tuned now")
     args = ap.parse_args(argv)
     from . import capi
     capi.load()
@@ -157,6 +160,10 @@ def main(argv=None) -> int:
         entries.append(e)
         print(f"[autotune] n={n}: ch={e['ch']} ach={e['ach']} sweep {e['k32_sweep_us']} ade {e['k32_ade_us']} "
               f"({time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
+    if args.merge:
+        tuned = {e["n"] for e in entries}
+        entries = sorted([e for e in load_table(Path(args.out)) if e.get("n") not in tuned] + entries,
+                         key=lambda e: e["cells"])
     table = {"entries": entries}
     if args.batch:
         table["batch"] = tune_batch()

@@ -755,6 +755,7 @@ int fx_advance(fx_domain d, double t_end, double print_next, int32_t max_steps, 
     c->status &= ~4;
     c->ade_error = 0;
     if ((rc = push_ctl(d))) return rc;
+    const bool fp32 = d->desc.precision == FX_FP32_MIXED;
     int32_t done = 0;
     double dt_est = c->dtfl;
     double tim0 = c->tim;
@@ -769,9 +770,15 @@ int fx_advance(fx_domain d, double t_end, double print_next, int32_t max_steps, 
             if ((rc = launch_step(d))) return rc;
         if ((rc = pull_ctl(d))) return rc;
         done = c->steps;
-        if (c->halt || (c->status & 12) || c->ade_error) break;
+        // FP32: the last step's flip and halt are the next clock's, so a step
+        // that reached its print time ends the loop by pending_halt
+        if (c->halt || (fp32 && c->pending_halt) || (c->status & 12) || c->ade_error) break;
         dt_est = c->dtfl;
         tim0 = c->tim;
+    }
+    if (fp32) {                                    // the last step's flip and halt
+        if ((rc = fx32_launch_step_close(d))) return rc;
+        if ((rc = pull_ctl(d))) return rc;
     }
     info->tim = c->tim;
     info->dtfl = c->dtfl;

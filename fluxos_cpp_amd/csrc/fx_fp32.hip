@@ -821,20 +821,23 @@ __global__ __launch_bounds__(64) void k32_cfl(fx_params P, s32p S, fx_ctl *ctl)
 // evaluate the forcing, gate the solver, update the FP64 water ledger.
 //
 // clock_fold: one block's max wave speed / max hpall / weir-volume sum over partials
-// [lo, hi), in a fixed order (the weir entries are consumed: set to 0).
+// [lo, hi), in a fixed order (the weir entries are consumed: set to 0).  All
+// loads come before the stores, so a thread's loads are in flight together.
 __device__ inline void clock_fold(double *partials, int lo, int hi, double &a, double &b, double &c, double &m)
 {
     const int tid = threadIdx.x;
     a = 0.0; b = 0.0; c = 0.0; m = 0.0;
+#pragma unroll 4
     for (int i = lo + tid; i < hi; i += blockDim.x) {
-        double *q = partials + FX32_NPV * (size_t)i;
-        a = fmax(a, q[0]);
-        b = fmax(b, q[1]);
-        c += q[2];
-        m += q[3];
-        q[2] = 0.0;
-        q[3] = 0.0;
+        const double2 *q = reinterpret_cast<const double2 *>(partials + FX32_NPV * (size_t)i);
+        const double2 x = q[0], y = q[1];
+        a = fmax(a, x.x);
+        b = fmax(b, x.y);
+        c += y.x;
+        m += y.y;
     }
+    for (int i = lo + tid; i < hi; i += blockDim.x)
+        reinterpret_cast<double2 *>(partials + FX32_NPV * (size_t)i)[1] = make_double2(0.0, 0.0);
     __shared__ double sm[4][16];
     a = wmax(a); b = wmax(b); c = wsum(c); m = wsum(m);
     if ((tid & 63) == 0) { sm[0][tid >> 6] = a; sm[1][tid >> 6] = b; sm[2][tid >> 6] = c; sm[3][tid >> 6] = m; }
@@ -858,55 +861,124 @@ __device__ inline void clock_body(const fx_params &P, double *partials, int nblk
     clock_finish(P, ctl, a, b, c, m);
 }
 
-// one domain: the partials are folded by several blocks in fixed chunks into
-// part2; the last block to finish (ticket) folds part2 in block order and
-// runs the clock, so the result does not depend on block timing
+// The end of a step (the buffer flip and the print / end / NaN halt), as the
+// next clock finds it: the device loop leaves it to the step after, one
+// launch fewer per step.  Returns whether the loop goes on; it also halts at
+// the end time (main.cpp:271).
+__device__ inline bool step_close_and_check(fx_ctl &C)
+{
+    if (C.flip_pending) {
+        C.flip_pending = 0;
+        C.cur ^= 1;
+        C.halt = C.pending_halt | ((C.status >> 2) & 1);      // halt on NaN/Inf
+    }
+    if (C.halt) return false;
+    if (C.tim > C.t_end) {
+        C.halt = 1;
+        return false;
+    }
+    return true;
+}
+
+// the control block into / out of a block's shared copy, word by word
+__device__ inline void ctl_load(fx_ctl &C, const fx_ctl *ctl)
+{
+    static_assert(sizeof(fx_ctl) % 8 == 0, "fx_ctl is copied in 8-byte words");
+    const uint64_t *g = reinterpret_cast<const uint64_t *>(ctl);
+    uint64_t *l = reinterpret_cast<uint64_t *>(&C);
+    for (int k = threadIdx.x; k < (int)(sizeof(fx_ctl) / 8); k += blockDim.x) l[k] = g[k];
+}
+__device__ inline void ctl_store(fx_ctl *ctl, const fx_ctl &C)
+{
+    uint64_t *g = reinterpret_cast<uint64_t *>(ctl);
+    const uint64_t *l = reinterpret_cast<const uint64_t *>(&C);
+    for (int k = threadIdx.x; k < (int)(sizeof(fx_ctl) / 8); k += blockDim.x) g[k] = l[k];
+}
+
+// One domain.  Every block reads the control block and decides the same way
+// whether the loop goes on, before any partial is consumed; nothing writes
+// the control block before every block has read it.  With one block, it folds
+// every partial; with several, each folds a fixed chunk into part2 and the
+// last block to finish (ticket) folds part2 in block order.  Either way the
+// result does not depend on block timing.  The block that runs the clock
+// works on a shared copy of the control block and writes it back.
 __global__ __launch_bounds__(1024) void k32_clock(fx_params P, double *partials, int nblk, double *part2,
                                                   uint32_t *ticket, fx_ctl *ctl)
 {
-    if (ctl->halt) return;
-    if (ctl->tim > ctl->t_end) {                       // loop condition, main.cpp:271
-        if (blockIdx.x == 0 && threadIdx.x == 0) ctl->halt = 1;
+    __shared__ fx_ctl C;
+    __shared__ int go_s;
+    ctl_load(C, ctl);
+    __syncthreads();
+    if (threadIdx.x == 0) go_s = step_close_and_check(C);   // the previous step closed
+    __syncthreads();
+    if (!go_s) {
+        // written back once every block has read the control block: by the
+        // only block, or by the last to take a ticket
+        if (gridDim.x > 1) {
+            __shared__ int last0;
+            if (threadIdx.x == 0) last0 = atomicAdd(ticket, 1u) == gridDim.x - 1;
+            __syncthreads();
+            if (!last0) return;
+            if (threadIdx.x == 0) *ticket = 0;
+        }
+        ctl_store(ctl, C);
         return;
     }
     const int chunk = (nblk + (int)gridDim.x - 1) / (int)gridDim.x;
     const int lo = min(nblk, (int)blockIdx.x * chunk), hi = min(nblk, lo + chunk);
     double a, b, c, m;
     clock_fold(partials, lo, hi, a, b, c, m);
-    __shared__ int last;
-    if (threadIdx.x == 0) {
-        part2[FX32_NPV * blockIdx.x] = a;
-        part2[FX32_NPV * blockIdx.x + 1] = b;
-        part2[FX32_NPV * blockIdx.x + 2] = c;
-        part2[FX32_NPV * blockIdx.x + 3] = m;
-        __threadfence();
-        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    // second level, by the whole block: the other blocks' part2 entries are
-    // read past this CU's cache (volatile), one entry per thread
-    __threadfence();
-    double A = 0.0, B = 0.0, Cw = 0.0, Mw = 0.0;
-    for (int k = threadIdx.x; k < (int)gridDim.x; k += blockDim.x) {
-        const volatile double *q = part2 + FX32_NPV * (size_t)k;
-        A = fmax(A, q[0]);
-        B = fmax(B, q[1]);
-        Cw += q[2];
-        Mw += q[3];
-    }
-    __shared__ double sm2[4][16];
-    A = wmax(A); B = wmax(B); Cw = wsum(Cw); Mw = wsum(Mw);
-    const int tid = threadIdx.x;
-    if ((tid & 63) == 0) { sm2[0][tid >> 6] = A; sm2[1][tid >> 6] = B; sm2[2][tid >> 6] = Cw; sm2[3][tid >> 6] = Mw; }
-    __syncthreads();
-    if (tid == 0) {
-        for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
-            A = fmax(A, sm2[0][w]); B = fmax(B, sm2[1][w]); Cw += sm2[2][w]; Mw += sm2[3][w];
+    if (gridDim.x > 1) {
+        __shared__ int last;
+        if (threadIdx.x == 0) {
+            part2[FX32_NPV * blockIdx.x] = a;
+            part2[FX32_NPV * blockIdx.x + 1] = b;
+            part2[FX32_NPV * blockIdx.x + 2] = c;
+            part2[FX32_NPV * blockIdx.x + 3] = m;
+            __threadfence();
+            last = atomicAdd(ticket, 1u) == gridDim.x - 1;
         }
-        *ticket = 0;
+        __syncthreads();
+        if (!last) return;
+        // second level, by the whole block: the other blocks' part2 entries are
+        // read past this CU's cache (volatile), one entry per thread
+        __threadfence();
+        double A = 0.0, B = 0.0, Cw = 0.0, Mw = 0.0;
+        for (int k = threadIdx.x; k < (int)gridDim.x; k += blockDim.x) {
+            const volatile double *q = part2 + FX32_NPV * (size_t)k;
+            A = fmax(A, q[0]);
+            B = fmax(B, q[1]);
+            Cw += q[2];
+            Mw += q[3];
+        }
+        __shared__ double sm2[4][16];
+        A = wmax(A); B = wmax(B); Cw = wsum(Cw); Mw = wsum(Mw);
+        const int tid = threadIdx.x;
+        if ((tid & 63) == 0) { sm2[0][tid >> 6] = A; sm2[1][tid >> 6] = B; sm2[2][tid >> 6] = Cw; sm2[3][tid >> 6] = Mw; }
+        __syncthreads();
+        if (tid == 0) {
+            for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+                A = fmax(A, sm2[0][w]); B = fmax(B, sm2[1][w]); Cw += sm2[2][w]; Mw += sm2[3][w];
+            }
+            *ticket = 0;
+        }
+        a = A; b = B; c = Cw; m = Mw;
     }
-    clock_finish(P, ctl, A, B, Cw, Mw);
+    clock_finish(P, &C, a, b, c, m);                   // thread 0 holds the folded values
+    if (threadIdx.x == 0) C.flip_pending = 1;          // this step's flip: the next clock's
+    __syncthreads();
+    ctl_store(ctl, C);
+}
+
+// the flip and halt of a device loop's last step, which the clock of a
+// following step would do: once at the end of fx_advance
+__global__ void k32_flip_end(fx_ctl *ctl)
+{
+    if (ctl->flip_pending) {
+        ctl->flip_pending = 0;
+        ctl->cur ^= 1;
+        ctl->halt = ctl->pending_halt | ((ctl->status >> 2) & 1);     // halt on NaN/Inf
+    }
 }
 
 // batched: one block per member
@@ -938,15 +1010,6 @@ __global__ __launch_bounds__(256) void k32_wintra(fx_params P, a32p A, double *s
         A.mass[nxt][k][i] += (float)(deltam / P.arbase);
     }
     soil[i] = m;
-}
-
-// end of a step: the buffer flip and the print / end halt (k_step_end)
-__global__ void k32_flip_end(fx_ctl *ctl)
-{
-    if (!ctl->halt) {
-        ctl->cur ^= 1;
-        ctl->halt = ctl->pending_halt | ((ctl->status >> 2) & 1);     // halt on NaN/Inf
-    }
 }
 
 // batched end of step: the buffer flip and the print/end halt of every
@@ -1360,11 +1423,18 @@ int fx32_launch_wintra(fx_domain d)
     return fx_hip_check(hipGetLastError(), "k32_wintra");
 }
 
+// One loop iteration of the device loop: its clock (which also closes the
+// previous iteration), the step sweep, the fast ADE and WINTRA.  The
+// iteration's own flip and halt are left to the next clock, or to
+// fx32_launch_step_close at the end of the loop.
 int fx32_launch_step(fx_domain d)
 {
     fx_params P = fx_make_params(d);
     s32p S = make_s32p(d);
-    const int nfold = (int)std::min<int64_t>(FX32_FOLD_MAX, std::max<int64_t>(1, (d->s32.npart + 1023) / 1024));
+    // one block folds up to 16 partials per thread; past that, blocks of 1024
+    const int nfold = d->s32.npart <= 16 * 1024
+                          ? 1
+                          : (int)std::min<int64_t>(FX32_FOLD_MAX, (d->s32.npart + 1023) / 1024);
     hipLaunchKernelGGL(k32_clock, dim3(nfold), dim3(1024), 0, d->stream, P, S.partials,
                        (int)d->s32.npart, d->s32.part2, d->s32.ticket, d->ctl);
     FX_HIP(hipGetLastError());
@@ -1384,6 +1454,12 @@ int fx32_launch_step(fx_domain d)
         int rc = fx32_launch_wintra(d);
         if (rc) return rc;
     }
+    return FX_OK;
+}
+
+// the flip and halt of the device loop's last iteration
+int fx32_launch_step_close(fx_domain d)
+{
     hipLaunchKernelGGL(k32_flip_end, dim3(1), dim3(1), 0, d->stream, d->ctl);
     return fx_hip_check(hipGetLastError(), "k32_flip_end");
 }

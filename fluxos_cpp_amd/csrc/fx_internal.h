@@ -33,6 +33,7 @@ struct fx_ctl {
     int32_t gate;        // hpall != 0 for the current step
     int32_t halt;        // steps are no-ops while set
     int32_t pending_halt;
+    int32_t flip_pending;// FP32 device loop: the step's buffer flip is left to the next clock
     int32_t steps;       // iterations run since the host last cleared them
     int32_t solver_steps;
     int32_t status;      // bit2: non-finite value seen
@@ -163,6 +164,7 @@ int fx32_launch_wintra(fx_domain d);
 
 int fx32_init_state(fx_domain d);
 int fx32_launch_step(fx_domain d);
+int fx32_launch_step_close(fx_domain d);
 int fx32_launch_cfl_initial(fx_domain d);
 int fx32_get_field(fx_domain d, int field, void *host);
 int fx32_set_field(fx_domain d, int field, const void *host);
