@@ -276,11 +276,38 @@ __device__ inline void a_column(const actx &X, awin<NC> &W, int c, bool upd, boo
     }
 }
 
+// Record b of the fold of the step sweep's CFL partials for the next clock:
+// max wave speed, max depth, weir volume and melt area over the partials of
+// chunk b, in a fixed order, by the block's one wave.  The clock reads the
+// records after this launch has ended.
+__device__ inline void fold_record(const a32fold &F, int b)
+{
+    const int chunk = (F.npart + F.nrec - 1) / F.nrec;
+    const int lo = min(F.npart, b * chunk), hi = min(F.npart, lo + chunk);
+    double a = 0.0, h = 0.0, w = 0.0, m = 0.0;
+    for (int i = lo + (int)threadIdx.x; i < hi; i += 64) {
+        const double2 *q = reinterpret_cast<const double2 *>(F.partials + FX32_NPV * (size_t)i);
+        const double2 x = q[0], y = q[1];
+        a = fmax(a, x.x);
+        h = fmax(h, x.y);
+        w += y.x;
+        m += y.y;
+    }
+    a = wmax(a); h = wmax(h); w = wsum(w); m = wsum(m);
+    if (threadIdx.x == 0) {
+        double2 *r = reinterpret_cast<double2 *>(F.rec + FX32_NPV * (size_t)b);
+        r[0] = make_double2(a, h);
+        r[1] = make_double2(w, m);
+    }
+}
+
 // one block of the sweep: strip bx (rows), column block by
 template <int NC>
-__device__ inline void ade_tile(const fx_params &P, const a32p &A, fx_ctl *ctl, int bx, int by)
+__device__ inline void ade_tile(const fx_params &P, const a32p &A, fx_ctl *ctl, int bx, int by,
+                                const a32fold &F)
 {
     if (ctl->halt) return;
+    if (F.rec && (int)blockIdx.x < F.nrec) fold_record(F, (int)blockIdx.x);
     const int cur = ctl->cur;
     actx X;
     X.P = &P;
@@ -365,10 +392,11 @@ __device__ inline void ade_tile(const fx_params &P, const a32p &A, fx_ctl *ctl, 
 }
 
 template <int NC>
-__global__ __launch_bounds__(64) void k32_ade_sweep(fx_params P, a32p A, fx_ctl *ctl, int gx, int n, int chunk)
+__global__ __launch_bounds__(64) void k32_ade_sweep(fx_params P, a32p A, fx_ctl *ctl, int gx, int n, int chunk,
+                                                    a32fold F)
 {
     const int t = fx_xcd_tile((int)blockIdx.x, n, chunk);
-    ade_tile<NC>(P, A, ctl, t % gx, t / gx);
+    ade_tile<NC>(P, A, ctl, t % gx, t / gx, F);
 }
 
 // batched: block b runs block FX_BMAP_BLK(amap[b]) of member FX_BMAP_SLOT(amap[b]);
@@ -382,15 +410,16 @@ __global__ __launch_bounds__(64) void k32b_ade_sweep(const fx_bslot *slots, cons
     const int blk = (int)FX_BMAP_BLK(m);
     a32p A = B.A;
     A.k0 = k0;
-    ade_tile<NC>(B.P, A, B.ctl, blk % B.agx, blk / B.agx);
+    const a32fold F{};
+    ade_tile<NC>(B.P, A, B.ctl, blk % B.agx, blk / B.agx, F);
 }
 
 template <int NC>
-void launch_nc(fx_domain d, const fx_params &P, a32p A, dim3 g)
+void launch_nc(fx_domain d, const fx_params &P, a32p A, dim3 g, const a32fold &F)
 {
     const int n = (int)(g.x * g.y);
     hipLaunchKernelGGL(k32_ade_sweep<NC>, dim3(n), dim3(64), 0, d->stream, P, A, d->ctl, (int)g.x, n,
-                       fx32_xcd_chunk((int)g.x, n));
+                       fx32_xcd_chunk((int)g.x, n), F);
 }
 
 inline dim3 ade_grid(const fx_params &P)
@@ -439,7 +468,9 @@ int fx32b_launch_ade(hipStream_t st, const fx_bslot *slots, const uint32_t *amap
     return fx_hip_check(hipGetLastError(), "k32b_ade_sweep");
 }
 
-int fx32_launch_ade(fx_domain d)
+// fold: the first launch also folds the step sweep's CFL partials into the
+// records the next clock reads
+int fx32_launch_ade(fx_domain d, bool fold)
 {
     fx_params P = fx_make_params(d);
     a32p A = make_a32p(d);
@@ -447,11 +478,18 @@ int fx32_launch_ade(fx_domain d)
     fx_prof_begin(d, FX_K_ADE32);
     for (int k0 = 0; k0 < P.nchem; k0 += ANC) {
         A.k0 = k0;
+        a32fold F{};
+        if (fold && k0 == 0) {
+            F.partials = d->s32.partials;
+            F.rec = d->s32.rec;
+            F.npart = (int)d->s32.npart;
+            F.nrec = std::min<int>(FX32_REC, (int)(g.x * g.y));
+        }
         switch (std::min(ANC, P.nchem - k0)) {
-        case 1: launch_nc<1>(d, P, A, g); break;
-        case 2: launch_nc<2>(d, P, A, g); break;
-        case 3: launch_nc<3>(d, P, A, g); break;
-        default: launch_nc<4>(d, P, A, g); break;
+        case 1: launch_nc<1>(d, P, A, g, F); break;
+        case 2: launch_nc<2>(d, P, A, g, F); break;
+        case 3: launch_nc<3>(d, P, A, g, F); break;
+        default: launch_nc<4>(d, P, A, g, F); break;
         }
     }
     fx_prof_end(d, FX_K_ADE32);

@@ -302,10 +302,12 @@ int launch_step64(fx_domain d)
     return fx_launch_step_end(d);
 }
 
-int launch_step(fx_domain d)
+// first: the call's first iteration (FP32: no fast ADE of this call has
+// folded the CFL partials yet)
+int launch_step(fx_domain d, bool first)
 {
     if (d->desc.precision == FX_FP64_PARITY) return launch_step64(d);
-    return fx32_launch_step(d);                    // ends with the flip and halt
+    return fx32_launch_step(d, first);
 }
 
 } // namespace
@@ -399,7 +401,7 @@ int fx_domain_destroy(fx_domain d)
     for (int k = 0; k < FX_MAX_CHEM; k++) dfree(s.conc[k]);
     fx_state32 &t = d->s32;
     void *ptrs32[] = {t.d[0], t.d[1], t.q[0], t.q[1], t.zb, t.f6, t.ff,
-                      t.flags, t.tw, t.partials, t.part2, t.ticket};
+                      t.flags, t.tw, t.partials, t.part2, t.ticket, t.rec};
     for (void *p : ptrs32) dfree(p);
     for (int k = 0; k < FX_MAX_CHEM; k++) { dfree(t.mass[0][k]); dfree(t.mass[1][k]); }
     for (auto &r : d->prof.pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
@@ -767,7 +769,7 @@ int fx_advance(fx_domain d, double t_end, double print_next, int32_t max_steps, 
         }
         n = std::min(n, max_steps - done);
         for (int i = 0; i < n; i++)
-            if ((rc = launch_step(d))) return rc;
+            if ((rc = launch_step(d, done == 0 && i == 0))) return rc;
         if ((rc = pull_ctl(d))) return rc;
         done = c->steps;
         // FP32: the last step's flip and halt are the next clock's, so a step

@@ -37,24 +37,9 @@
 
 namespace {
 
-// wave reductions, and the clock of one loop iteration (main.cpp:271-325,
-// 385-387) from the folded CFL partials of the previous step sweep: used by
-// k32_clock (one domain) and k32b_clock (batch members)
-__device__ inline double wmin(double v)
-{
-    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ inline double wmax(double v)
-{
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ inline double wsum(double v)
-{
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
+// the clock of one loop iteration (main.cpp:271-325, 385-387) from the
+// folded CFL partials of the previous step sweep: used by k32_clock (one
+// domain) and k32b_clock (batch members)
 
 // the CFL time step from the largest wave speed of the domain (9e10 s, the
 // reference's initial bound, where no cell is above hdry)
@@ -142,6 +127,39 @@ __device__ inline void clock_finish(const fx_params &P, fx_ctl *ctl, double a, d
     __syncthreads();
     const int mrow = P.meteo ? fx_forcing_row_block(P.meteo, P.meteo_nrows, P.meteo_ncols, tim_new) : -1;
     const int irow = P.inflow ? fx_forcing_row_block(P.inflow, P.inflow_nrows, P.inflow_ncols, tim_new) : -1;
+    if (threadIdx.x == 0) clock_forcing(P, ctl, mrow, irow);
+}
+
+// The forcing row for tim with thread t holding the time of row t (tables of
+// at most blockDim.x rows, loaded before the clock knows tim): the smallest
+// row whose time exceeds tim, -1 if none.  All threads call it.
+__device__ inline int forcing_row_held(float t_row, int nrows, double tim)
+{
+    __shared__ int best;
+    if (threadIdx.x == 0) best = 0x7fffffff;
+    __syncthreads();
+    if ((int)threadIdx.x < nrows && (double)t_row > tim) atomicMin(&best, (int)threadIdx.x);
+    __syncthreads();
+    const int r = best;
+    __syncthreads();
+    return r == 0x7fffffff ? -1 : r;
+}
+
+// clock_finish with the forcing tables' times already held (mt, it: row
+// threadIdx.x of the melt and inflow tables; held = both tables fit)
+__device__ inline void clock_finish_held(const fx_params &P, fx_ctl *ctl, double a, double b, double c, double m,
+                                         bool held, float mt, float it)
+{
+    if (!held) {
+        clock_finish(P, ctl, a, b, c, m);
+        return;
+    }
+    __shared__ double tim_new2;
+    if (threadIdx.x == 0) clock_advance(P, ctl, a, b, c, m);
+    if (threadIdx.x == 0) tim_new2 = ctl->tim;
+    __syncthreads();
+    const int mrow = P.meteo ? forcing_row_held(mt, P.meteo_nrows, tim_new2) : -1;
+    const int irow = P.inflow ? forcing_row_held(it, P.inflow_nrows, tim_new2) : -1;
     if (threadIdx.x == 0) clock_forcing(P, ctl, mrow, irow);
 }
 
@@ -897,16 +915,24 @@ __device__ inline void ctl_store(fx_ctl *ctl, const fx_ctl &C)
 
 // One domain.  Every block reads the control block and decides the same way
 // whether the loop goes on, before any partial is consumed; nothing writes
-// the control block before every block has read it.  With one block, it folds
-// every partial; with several, each folds a fixed chunk into part2 and the
-// last block to finish (ticket) folds part2 in block order.  Either way the
-// result does not depend on block timing.  The block that runs the clock
-// works on a shared copy of the control block and writes it back.
+// the control block before every block has read it.  With nrec > 0 (one
+// block) the fast ADE launch of the previous step has folded the partials
+// into nrec records, and the block folds those.  Otherwise one block folds
+// every partial, or several each fold a fixed chunk into part2 and the last
+// block to finish (ticket) folds part2 in block order.  Either way the result
+// does not depend on block timing.  The block that runs the clock works on a
+// shared copy of the control block and writes it back; the forcing tables'
+// times are loaded with it when they fit one entry per thread.
 __global__ __launch_bounds__(1024) void k32_clock(fx_params P, double *partials, int nblk, double *part2,
-                                                  uint32_t *ticket, fx_ctl *ctl)
+                                                  uint32_t *ticket, const double *rec, int nrec, fx_ctl *ctl)
 {
     __shared__ fx_ctl C;
     __shared__ int go_s;
+    const bool held = (!P.meteo || P.meteo_nrows <= (int)blockDim.x) &&
+                      (!P.inflow || P.inflow_nrows <= (int)blockDim.x);
+    float mt = 0.f, it = 0.f;
+    if (held && P.meteo && (int)threadIdx.x < P.meteo_nrows) mt = P.meteo[(size_t)threadIdx.x * P.meteo_ncols];
+    if (held && P.inflow && (int)threadIdx.x < P.inflow_nrows) it = P.inflow[(size_t)threadIdx.x * P.inflow_ncols];
     ctl_load(C, ctl);
     __syncthreads();
     if (threadIdx.x == 0) go_s = step_close_and_check(C);   // the previous step closed
@@ -924,10 +950,26 @@ __global__ __launch_bounds__(1024) void k32_clock(fx_params P, double *partials,
         ctl_store(ctl, C);
         return;
     }
-    const int chunk = (nblk + (int)gridDim.x - 1) / (int)gridDim.x;
-    const int lo = min(nblk, (int)blockIdx.x * chunk), hi = min(nblk, lo + chunk);
     double a, b, c, m;
-    clock_fold(partials, lo, hi, a, b, c, m);
+    if (nrec > 0) {
+        // the records (their partials are rewritten by the next sweep, so
+        // none needs clearing)
+        a = 0.0; b = 0.0; c = 0.0; m = 0.0;
+        if ((int)threadIdx.x < nrec) {
+            const double2 *q = reinterpret_cast<const double2 *>(rec + FX32_NPV * (size_t)threadIdx.x);
+            const double2 x = q[0], y = q[1];
+            a = x.x; b = x.y; c = y.x; m = y.y;
+        }
+        a = wmax(a); b = wmax(b); c = wsum(c); m = wsum(m);  // nrec <= 64: the first wave
+        __shared__ double r4[4];
+        if (threadIdx.x == 0) { r4[0] = a; r4[1] = b; r4[2] = c; r4[3] = m; }
+        __syncthreads();
+        a = r4[0]; b = r4[1]; c = r4[2]; m = r4[3];
+    } else {
+        const int chunk = (nblk + (int)gridDim.x - 1) / (int)gridDim.x;
+        const int lo = min(nblk, (int)blockIdx.x * chunk), hi = min(nblk, lo + chunk);
+        clock_fold(partials, lo, hi, a, b, c, m);
+    }
     if (gridDim.x > 1) {
         __shared__ int last;
         if (threadIdx.x == 0) {
@@ -964,7 +1006,7 @@ __global__ __launch_bounds__(1024) void k32_clock(fx_params P, double *partials,
         }
         a = A; b = B; c = Cw; m = Mw;
     }
-    clock_finish(P, &C, a, b, c, m);                   // thread 0 holds the folded values
+    clock_finish_held(P, &C, a, b, c, m, held, mt, it);   // thread 0 holds the folded values
     if (threadIdx.x == 0) C.flip_pending = 1;          // this step's flip: the next clock's
     __syncthreads();
     ctl_store(ctl, C);
@@ -1153,6 +1195,7 @@ int fx32_init_state(fx_domain d)
     if ((rc = dalloc32(d, &t.partials, FX32_NPV * cap))) return rc;
     if ((rc = dalloc32(d, &t.part2, FX32_NPV * FX32_FOLD_MAX))) return rc;
     if ((rc = dalloc32(d, &t.ticket, 1))) return rc;
+    if ((rc = dalloc32(d, &t.rec, FX32_NPV * FX32_REC))) return rc;
     return FX_OK;
 }
 
@@ -1427,18 +1470,21 @@ int fx32_launch_wintra(fx_domain d)
 // previous iteration), the step sweep, the fast ADE and WINTRA.  The
 // iteration's own flip and halt are left to the next clock, or to
 // fx32_launch_step_close at the end of the loop.
-int fx32_launch_step(fx_domain d)
+int fx32_launch_step(fx_domain d, bool first)
 {
     fx_params P = fx_make_params(d);
     s32p S = make_s32p(d);
-    // one block folds up to 16 partials per thread; past that, blocks of 1024
-    const int nfold = d->s32.npart <= 16 * 1024
+    const int ade_on = (d->desc.ade_solver && d->desc.ade_mode == FX_ADE_FAST) ? 1 : 0;
+    // after an iteration of this loop the fast ADE has folded the partials
+    // into records; else one block folds up to 16 partials per thread, and
+    // past that blocks of 1024 fold chunks
+    const int nrec = (!first && ade_on && P.nchem >= 1) ? std::min(FX32_REC, fx32_ade_blocks(d)) : 0;
+    const int nfold = (nrec > 0 || d->s32.npart <= 16 * 1024)
                           ? 1
                           : (int)std::min<int64_t>(FX32_FOLD_MAX, (d->s32.npart + 1023) / 1024);
     hipLaunchKernelGGL(k32_clock, dim3(nfold), dim3(1024), 0, d->stream, P, S.partials,
-                       (int)d->s32.npart, d->s32.part2, d->s32.ticket, d->ctl);
+                       (int)d->s32.npart, d->s32.part2, d->s32.ticket, (const double *)d->s32.rec, nrec, d->ctl);
     FX_HIP(hipGetLastError());
-    const int ade_on = (d->desc.ade_solver && d->desc.ade_mode == FX_ADE_FAST) ? 1 : 0;
     fx_prof_begin(d, FX_K_STEP32);
     const dim3 g = grid32(P);
     const int n = (int)(g.x * g.y);
@@ -1447,7 +1493,7 @@ int fx32_launch_step(fx_domain d)
     fx_prof_end(d, FX_K_STEP32);
     FX_HIP(hipGetLastError());
     if (ade_on) {
-        int rc = fx32_launch_ade(d);
+        int rc = fx32_launch_ade(d, true);
         if (rc) return rc;
     }
     if (d->wintra) {

@@ -81,6 +81,7 @@ struct fx_state32 {
     double *partials;          // 3 per step-kernel block (dt, hpall, weir volume)
     double *part2;             // 3 per clock block (second level of the fold)
     uint32_t *ticket;          // last-block counter of the clock
+    double *rec;               // FX32_REC folds of the partials, by the fast ADE launch
     int64_t npart;
 };
 
@@ -163,13 +164,13 @@ int fx_inflow_cell_ok(fx_domain d);
 int fx32_launch_wintra(fx_domain d);
 
 int fx32_init_state(fx_domain d);
-int fx32_launch_step(fx_domain d);
+int fx32_launch_step(fx_domain d, bool first);
 int fx32_launch_step_close(fx_domain d);
 int fx32_launch_cfl_initial(fx_domain d);
 int fx32_get_field(fx_domain d, int field, void *host);
 int fx32_set_field(fx_domain d, int field, const void *host);
 int fx32_launch_volume(fx_domain d, double *vol_out);
-int fx32_launch_ade(fx_domain d);
+int fx32_launch_ade(fx_domain d, bool fold);
 int fx32_pending_weir(fx_domain d, double *vol);
 int fx32_launch_solute(fx_domain d, int k, double *mass_out);
 // water-volume block partials of fx_mass_balance into a caller's buffer

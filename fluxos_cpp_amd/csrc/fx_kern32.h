@@ -28,6 +28,33 @@ struct a32p {
     int k0;                   // first solute of this launch
 };
 
+// The fold of a single domain's CFL partials into FX32_REC records by the
+// first FX32_REC blocks of the fast ADE launch that follows the step sweep,
+// for the next iteration's clock (rec == NULL: no fold)
+struct a32fold {
+    const double *partials;
+    double *rec;
+    int32_t npart, nrec;
+};
+#define FX32_REC 64
+
+// wave reductions of one double (64 lanes, butterfly: every lane gets the result)
+__device__ inline double wmin(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ inline double wmax(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ inline double wsum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
 // tiling of the FP32 kernels
 // The FP32 kernels address a cell's element as a uniform base plus a 32-bit
 // byte offset (cell index times element size), which the hardware takes as a
