@@ -48,9 +48,9 @@ struct apref {
     double dold, dnew;
     float fe, fn, mult;
     float m[NC];
-    uint32_t f;               // flag byte, plus AF_INFLOW
+    uint8_t f;                // flag byte (kept a byte: a widening after the
+                              // load would wait for the load there)
 };
-#define AF_INFLOW 0x100u      // the cell takes the inflow point source
 
 // per-cell geometry of one column, shared by the solutes
 struct ageo {
@@ -65,7 +65,7 @@ struct actx {
     const float *m_in[ANC];
     float *m_out[ANC];
     double meteoi, inflowi;
-    uint32_t inflow_p;        // linear index of the inflow cell, ~0u = none
+    int inflow_c;             // the inflow cell's column if it is in this lane's row, else -1
     float cm[ANC], cin[ANC], kdiff[ANC];
     float hdry, dx, area_dt, dt_area;
     int row, mcols;
@@ -88,17 +88,18 @@ __device__ inline void a_load(const actx &X, int c, apref<NC> &a)
     a.fe = ff.x;
     a.fn = ff.y;
     a.mult = X.P->melt_mult ? at_off(X.P->melt_mult, o4) : 1.0f;
-    if (p == X.inflow_p) a.f |= AF_INFLOW;
 #pragma unroll
     for (int k = 0; k < NC; k++) a.m[k] = at_off(X.m_in[k], o4);
 }
 
-// derived values of a freshly loaded cell: geometry, and per solute the
-// pass-1 concentration c1 (craw where the cell was dry before the step, 0 where
-// dry now) and the raw concentration cr after the melt mixing.  Without
-// transport c1 carries the mixed mass itself, which is the update.
+// derived values of a freshly loaded cell of column col: geometry, and per
+// solute the pass-1 concentration c1 (craw where the cell was dry before the
+// step, 0 where dry now) and the raw concentration cr after the melt mixing.
+// Without transport c1 carries the mixed mass itself, which is the update.
+// The inflow cell is found by its column, not by a bit of the prefetched
+// flags: a test on a loaded value right after the load would wait for it there.
 template <int NC>
-__device__ inline void a_derive(const actx &X, const apref<NC> &a, ageo &g, float *c1, float *cr)
+__device__ inline void a_derive(const actx &X, const apref<NC> &a, int col, ageo &g, float *c1, float *cr)
 {
     const fx_params &P = *X.P;
     const bool inner = a.f & FXF_INNER;
@@ -109,7 +110,7 @@ __device__ inline void a_derive(const actx &X, const apref<NC> &a, ageo &g, floa
     const bool wp = inner && !(fmax(0.0, a.dold) < P.hdry) && !weir;
     double dm = a.dold + melt;
     float inf = 0.f;                                   // inflow depth carrying solute
-    if (a.f & AF_INFLOW) {                             // add_inflow (common.cpp:324-376)
+    if (col == X.inflow_c) {                           // add_inflow (common.cpp:324-376)
         if (dm <= 0.0) dm = 0.0;
         if (dm > 0.0) inf = (float)X.inflowi;          // solute arrives only into water
         dm += X.inflowi;
@@ -317,7 +318,6 @@ __device__ inline void ade_tile(const fx_params &P, const a32p &A, fx_ctl *ctl, 
     X.transport = ctl->gate && ctl->it > 1;
     X.meteoi = P.meteo ? ctl->meteoi : 0.0;
     X.inflowi = ctl->inflowi;
-    X.inflow_p = P.inflow ? (uint32_t)P.inflow_p : ~0u;
     const float dt = (float)ctl->dtfl;
     X.hdry = (float)P.hdry;
     X.dx = (float)P.dxy;
@@ -338,6 +338,9 @@ __device__ inline void ade_tile(const fx_params &P, const a32p &A, fx_ctl *ctl, 
     X.row = r0 - 3 + lane;
     X.row_ok = X.row >= 0 && X.row < P.ld;
     X.mcols = P.ncols + 2;
+    X.inflow_c = -1;                                  // the inflow cell is interior (host-checked)
+    if (P.inflow && X.row == (int)((uint32_t)P.inflow_p % (uint32_t)P.ld))
+        X.inflow_c = (int)((uint32_t)P.inflow_p / (uint32_t)P.ld);
     X.ld = (uint32_t)P.ld;
     const bool own = lane >= 3 && lane <= 3 + AOWN - 1 && X.row <= P.nrows;
 
@@ -346,15 +349,15 @@ __device__ inline void ade_tile(const fx_params &P, const a32p &A, fx_ctl *ctl, 
     bool bad = false;
     // prologue: window = columns c0-3 .. c0; the N-face fluxes of column
     // c0-2 (before limiting) and of column c0-1 (both) feed column c0
-    a_load<NC>(X, c0 - 3, a); a_derive<NC>(X, a, W.g[0], W.c1[0], W.cr[0]);
-    a_load<NC>(X, c0 - 2, a); a_derive<NC>(X, a, W.g[1], W.c1[1], W.cr[1]);
-    a_load<NC>(X, c0 - 1, a); a_derive<NC>(X, a, W.g[2], W.c1[2], W.cr[2]);
-    a_load<NC>(X, c0, a);     a_derive<NC>(X, a, W.g[3], W.c1[3], W.cr[3]);
+    a_load<NC>(X, c0 - 3, a); a_derive<NC>(X, a, c0 - 3, W.g[0], W.c1[0], W.cr[0]);
+    a_load<NC>(X, c0 - 2, a); a_derive<NC>(X, a, c0 - 2, W.g[1], W.c1[1], W.cr[1]);
+    a_load<NC>(X, c0 - 1, a); a_derive<NC>(X, a, c0 - 1, W.g[2], W.c1[2], W.cr[2]);
+    a_load<NC>(X, c0, a);     a_derive<NC>(X, a, c0, W.g[3], W.c1[3], W.cr[3]);
 #pragma unroll
     for (int k = 0; k < NC; k++) { W.pn_s[k] = 0.f; W.n_s[k] = 0.f; }
     a_column<NC>(X, W, c0 - 2, false, own, bad);
     a_shift<NC>(W);
-    a_load<NC>(X, c0 + 1, a); a_derive<NC>(X, a, W.g[3], W.c1[3], W.cr[3]);
+    a_load<NC>(X, c0 + 1, a); a_derive<NC>(X, a, c0 + 1, W.g[3], W.c1[3], W.cr[3]);
     a_column<NC>(X, W, c0 - 1, false, own, bad);
     if constexpr (NC >= 3) {
         // three or four solutes: one column of prefetch (a: c+2) keeps the
@@ -362,7 +365,7 @@ __device__ inline void ade_tile(const fx_params &P, const a32p &A, fx_ctl *ctl, 
         a_load<NC>(X, c0 + 2, a);
         for (int c = c0; c <= c_last; c++) {
             a_shift<NC>(W);
-            a_derive<NC>(X, a, W.g[3], W.c1[3], W.cr[3]);
+            a_derive<NC>(X, a, c + 2, W.g[3], W.c1[3], W.cr[3]);
             if (c + 3 <= c_last + 2) a_load<NC>(X, c + 3, a);
             a_column<NC>(X, W, c, true, own, bad);
         }
@@ -375,17 +378,17 @@ __device__ inline void ade_tile(const fx_params &P, const a32p &A, fx_ctl *ctl, 
     int c = c0;
     for (; c + 1 <= c_last; c += 2) {
         a_shift<NC>(W);
-        a_derive<NC>(X, a, W.g[3], W.c1[3], W.cr[3]);
+        a_derive<NC>(X, a, c + 2, W.g[3], W.c1[3], W.cr[3]);
         if (c + 4 <= c_last + 2) a_load<NC>(X, c + 4, a);
         a_column<NC>(X, W, c, true, own, bad);
         a_shift<NC>(W);
-        a_derive<NC>(X, b, W.g[3], W.c1[3], W.cr[3]);
+        a_derive<NC>(X, b, c + 3, W.g[3], W.c1[3], W.cr[3]);
         if (c + 5 <= c_last + 2) a_load<NC>(X, c + 5, b);
         a_column<NC>(X, W, c + 1, true, own, bad);
     }
     if (c <= c_last) {
         a_shift<NC>(W);
-        a_derive<NC>(X, a, W.g[3], W.c1[3], W.cr[3]);
+        a_derive<NC>(X, a, c + 2, W.g[3], W.c1[3], W.cr[3]);
         a_column<NC>(X, W, c, true, own, bad);
     }
     if (__any(bad) && lane == 0) atomicOr(&ctl->status, 4);

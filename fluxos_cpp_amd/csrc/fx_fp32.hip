@@ -423,9 +423,9 @@ struct pref {                 // one cell's HBM values in flight
     double d;
     double zb;
     float qx, qy, mult;
-    uint32_t f;               // flag byte, plus PF_INFLOW
+    uint8_t f;                // flag byte (kept a byte: a widening after the
+                              // load would wait for the load there)
 };
-#define PF_INFLOW 0x100u      // the cell takes the inflow point source
 
 // Three columns (c-1, c, c+1) of what the flux stencil reads from the S
 // neighbour (u; the wet flags are lane masks, cmask); columns c and c+1 of the rest, and the
@@ -475,6 +475,7 @@ struct sweep_ctx {
     consts32 K;
     double meteoi, inflowi, s_wet_prev;
     uint32_t inflow_p;        // linear index of the inflow cell, ~0u = none
+    int inflow_c;             // the inflow cell's column if it is in this lane's row, else -1
     float dxy_over_dt, kf, dtf, dtf1, hdryf;     // dtf1 = dt / 10
     int gate, ade_on, k, row, mcols;
     bool row_ok, own;
@@ -495,23 +496,25 @@ __device__ inline void load_col(const sweep_ctx &X, int c, pref &a)
     a.qx = q.x;
     a.qy = q.y;
     a.mult = X.P->melt_mult ? at_off(X.P->melt_mult, o4) : 1.0f;
-    if (p == X.inflow_p) a.f |= PF_INFLOW;
 }
 
-// L: this lane's cell of the column into ring slots SL (three-column arrays)
-// and par (two-column record): melt of this step, hydro loop 1, cell velocities
+// L: this lane's cell of column col into ring slots SL (three-column arrays)
+// and col & 1 (two-column record): melt of this step, hydro loop 1, cell
+// velocities.  The inflow cell is found by its column, not by a bit of the
+// prefetched flags: a test on a loaded value right after the load would wait
+// for it there.
 template <int SL>
-__device__ inline void store_col(const sweep_ctx &X, ring &R, cmask &M, const pref &a, int par, bool own_col,
+__device__ inline void store_col(const sweep_ctx &X, ring &R, cmask &M, const pref &a, int col, bool own_col,
                                  double &ms)
 {
-    const int i = X.k;
+    const int i = X.k, par = col & 1;
     const bool inner = a.f & FXF_INNER;
     // melt area that brings solute (cells holding water before the melt,
     // common.cpp:214-233), summed over the cells this lane owns
     if (own_col && X.own && inner && !(a.f & FXF_NODATA) && a.d > 0.0) ms += (double)a.mult;
     double d = inner ? a.d : 0.0;
     if (inner && !(a.f & FXF_NODATA)) d += X.meteoi * (double)a.mult;
-    if (a.f & PF_INFLOW) {                              // add_inflow after add_meteo (common.cpp:324-330)
+    if (col == X.inflow_c) {                            // add_inflow after add_meteo (common.cpp:324-330)
         if (d <= 0.0) d = 0.0;
         d += X.inflowi;
     }
@@ -692,7 +695,7 @@ __device__ inline void column_step(const sweep_ctx &X, ring &R, cmask &M, int c,
 {
     constexpr int SN = (SC + 1) % 3, SS = (SC + 2) % 3;
     __syncthreads();
-    store_col<SN>(X, R, M, a, (c + 1) & 1, c + 1 <= c_last, ms);   // column c+1 (prefetched last step)
+    store_col<SN>(X, R, M, a, c + 1, c + 1 <= c_last, ms);   // column c+1 (prefetched last step)
     if (c + 2 <= c_last + 1) load_col(X, c + 2, a);
     __syncthreads();
     flux_col<SC, SS, SN>(X, R, M, c);
@@ -733,6 +736,8 @@ __device__ inline void sweep_tile(const fx_params &P, const s32p &S, fx_ctl *ctl
     X.row = r0 - 2 + X.k;                             // this lane's row
     X.row_ok = X.row >= 0 && X.row < P.ld;
     X.own = X.k >= 2 && X.k <= NS - 2 && X.row <= P.nrows;
+    X.inflow_c = -1;                                  // the inflow cell is interior (host-checked)
+    if (P.inflow && X.row == (int)(X.inflow_p % (uint32_t)P.ld)) X.inflow_c = (int)(X.inflow_p / (uint32_t)P.ld);
     X.mcols = P.ncols + 2;
     X.ld = (uint32_t)P.ld;
     X.hdryf = (float)P.hdry;
@@ -759,9 +764,9 @@ __device__ inline void sweep_tile(const fx_params &P, const s32p &S, fx_ctl *ctl
     bool bad = false;
     // prologue: columns c0-2 (slot 1), c0-1 (slot 2), c0 (slot 0), fluxes of c0-1
     cmask M;
-    load_col(X, c0 - 2, a); store_col<1>(X, R, M, a, (c0 - 2) & 1, false, ms);
-    load_col(X, c0 - 1, a); store_col<2>(X, R, M, a, (c0 - 1) & 1, false, ms);
-    load_col(X, c0, a); store_col<0>(X, R, M, a, c0 & 1, true, ms);
+    load_col(X, c0 - 2, a); store_col<1>(X, R, M, a, c0 - 2, false, ms);
+    load_col(X, c0 - 1, a); store_col<2>(X, R, M, a, c0 - 1, false, ms);
+    load_col(X, c0, a); store_col<0>(X, R, M, a, c0, true, ms);
     load_col(X, c0 + 1, a);
     __syncthreads();
     flux_col<2, 1, 0>(X, R, M, c0 - 1);
