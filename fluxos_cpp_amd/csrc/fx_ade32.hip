@@ -105,9 +105,9 @@ __device__ inline void a_derive(const actx &X, const apref<NC> &a, int col, ageo
     const bool inner = a.f & FXF_INNER;
     const bool weir = a.f & FXF_WEIR;
     const double melt = (!inner || (a.f & FXF_NODATA)) ? 0.0 : X.meteoi * (double)a.mult;
-    const double hn = fmax(0.0, a.dnew);
-    const bool w = inner && !(hn < P.hdry) && !weir;
-    const bool wp = inner && !(fmax(0.0, a.dold) < P.hdry) && !weir;
+    // max(0, d) >= hdry > 0 is d >= hdry (a NaN depth is dry in both forms)
+    const bool w = inner && a.dnew >= P.hdry && !weir;
+    const bool wp = inner && a.dold >= P.hdry && !weir;
     double dm = a.dold + melt;
     float inf = 0.f;                                   // inflow depth carrying solute
     if (col == X.inflow_c) {                           // add_inflow (common.cpp:324-376)
@@ -117,7 +117,9 @@ __device__ inline void a_derive(const actx &X, const apref<NC> &a, int col, ageo
     }
     const float h0 = inner ? (float)fmax(0.0, dm) : 0.f;
     const float mlt = (inner && a.dold > 0.0) ? (float)melt : 0.f;   // solute arrives with melt only into water
-    g.h = inner ? (float)hn : 0.f;
+    // the depth and its square root are read only where the cell is wet
+    // (faces towards a dry cell carry nothing, and a dry cell computes none)
+    g.h = w ? (float)a.dnew : 0.f;
     g.sh = fx_sqrt(g.h);
     g.w = w;
     g.fe = inner ? a.fe : 0.f;
@@ -265,8 +267,10 @@ __device__ inline void a_column(const actx &X, awin<NC> &W, int c, bool upd, boo
                 float con = cp + (e_w - e + W.n_s[k] - n) * kup;
                 // bounds of pass 1: W and S already rescaled, E and N not yet (:51-62)
                 const float cs = W.c1[0][k], cn = W.cr[2][k];
-                con = fminf(fmaxf(fmaxf(cW, crE), fmaxf(cs, cn)), con);
-                con = fmaxf(fminf(fminf(cW, crE), fminf(cs, cn)), con);
+                // clamped to [lo, hi] (lo <= hi) by one median of three
+                const float hi = fmaxf(fmaxf(cW, crE), fmaxf(cs, cn));
+                const float lo = fminf(fminf(cW, crE), fminf(cs, cn));
+                con = __builtin_amdgcn_fmed3f(con, lo, hi);
                 out = con * gP.h;
             }
             bad |= X.transport && gP.w && !fx_finitef(out);   // reported as ctl->status bit 2
