@@ -409,7 +409,7 @@ __global__ __launch_bounds__(64) void k32_ade_sweep(fx_params P, a32p A, fx_ctl 
 // batched: block b runs block FX_BMAP_BLK(amap[b]) of member FX_BMAP_SLOT(amap[b]);
 // every member of a batch carries the same number of solutes
 template <int NC>
-__global__ __launch_bounds__(64) void k32b_ade_sweep(const fx_bslot *slots, const uint32_t *amap, int k0)
+__global__ __launch_bounds__(64) void k32b_ade_sweep(const fx_bslot *__restrict__ slots, const uint32_t *__restrict__ amap, int k0)
 {
     const uint32_t m = amap[blockIdx.x];
     const fx_bslot &B = slots[FX_BMAP_SLOT(m)];

@@ -551,9 +551,8 @@ __device__ inline void flux_col(const sweep_ctx &X, ring &R, const cmask &M, int
             if (!lane_bit((M.w[S0] << 1) | (M.w[S0] >> 1) | M.w[SS] | M.w[SN])) {
                 if (R.p[pc].raw[i] & FXF_STALE) {  // stale fluxes of an all-dry neighbourhood
                     const uint32_t p = (uint32_t)X.row + (uint32_t)c * X.ld;
-                    const float2 *s6 = reinterpret_cast<const float2 *>(X.S->f6 + (size_t)p * 6);
-                    const float2 a = s6[0], b = s6[1], e = s6[2];
-                    f[0] = a.x; f[1] = a.y; f[2] = b.x; f[3] = b.y; f[4] = e.x; f[5] = e.y;
+                    const FX_GLOBAL float *s6 = (const FX_GLOBAL float *)(X.S->f6 + (size_t)p * 6);
+                    for (int m = 0; m < 6; m++) f[m] = s6[m];
                 }
             } else {
                 solver_dry32(X.K, front_of(R, pc, pn, i), wetE, wetN, X.dxy_over_dt, f);
@@ -650,10 +649,8 @@ __device__ inline void update_col(const sweep_ctx &X, ring &R, const cmask &M, i
                 bool nz = false;
                 for (int m = 0; m < 6; m++) nz |= rflux(R, fs, m, i) != 0.f;
                 if (nz) {
-                    float2 *s6 = reinterpret_cast<float2 *>(S.f6 + (size_t)p * 6);
-                    s6[0] = make_float2(rflux(R, fs, 0, i), rflux(R, fs, 1, i));
-                    s6[1] = make_float2(rflux(R, fs, 2, i), rflux(R, fs, 3, i));
-                    s6[2] = make_float2(rflux(R, fs, 4, i), rflux(R, fs, 5, i));
+                    FX_GLOBAL float *s6 = (FX_GLOBAL float *)(S.f6 + (size_t)p * 6);
+                    for (int m = 0; m < 6; m++) s6[m] = rflux(R, fs, m, i);
                     nf |= FXF_STALE;
                 } else {
                     nf &= ~FXF_STALE;
@@ -802,7 +799,7 @@ __global__ __launch_bounds__(64) void k32_sweep(fx_params P, s32p S, fx_ctl *ctl
 }
 
 // batched: block b runs block FX_BMAP_BLK(tmap[b]) of member FX_BMAP_SLOT(tmap[b])
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k32b_sweep(const fx_bslot *slots, const uint32_t *tmap)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k32b_sweep(const fx_bslot *__restrict__ slots, const uint32_t *__restrict__ tmap)
 {
     __shared__ ring R;
     const uint32_t m = tmap[blockIdx.x];

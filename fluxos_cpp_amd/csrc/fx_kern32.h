@@ -61,15 +61,25 @@ __device__ inline double wsum(double v)
 // scalar base and a 32-bit vector offset.  Domains are limited to
 // FX32_MAX_CELLS so that the offset of an 8-byte element fits.
 #define FX32_MAX_CELLS ((int64_t)1 << 29)
+// The access goes through a global-memory pointer: every array the FP32
+// kernels address this way is device memory, and a pointer read from memory
+// (a batch member's slot) would otherwise be a generic one, whose loads count
+// on the LDS counter as well, so that every LDS wait would also wait for them.
+// (the host pass of these inline device functions sees plain pointers)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define FX_GLOBAL __attribute__((address_space(1)))
+#else
+#define FX_GLOBAL
+#endif
 template <typename T>
-__device__ inline T &at_off(T *base, uint32_t off)
+__device__ inline FX_GLOBAL T &at_off(T *base, uint32_t off)
 {
-    return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off);
+    return *(FX_GLOBAL T *)(reinterpret_cast<char *>(base) + off);
 }
 template <typename T>
-__device__ inline const T &at_off(const T *base, uint32_t off)
+__device__ inline const FX_GLOBAL T &at_off(const T *base, uint32_t off)
 {
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
+    return *(const FX_GLOBAL T *)(reinterpret_cast<const char *>(base) + off);
 }
 
 // finiteness by the exponent bits, which no floating-point option of the
