@@ -85,7 +85,7 @@ def tune(n: int, steps: int = 40, t_wet: float = 3600.0) -> dict:
             "workload": f"fractal {n}x{n}, 2 solutes, run on from t={t_wet:.0f}s, {steps} steps per shape, best of 2"}
 
 
-BATCH_CANDIDATES = ((8, 16), (8, 32), (8, 64), (12, 32), (16, 32), (20, 16))
+BATCH_CANDIDATES = ((8, 32), (8, 64), (12, 64), (16, 64), (23, 64))
 
 
 def tune_batch(count: int = 64, candidates=BATCH_CANDIDATES) -> dict:
