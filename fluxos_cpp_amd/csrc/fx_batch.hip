@@ -78,11 +78,51 @@ int grow(uint32_t **p, int64_t *cap, int64_t need)
     return FX_OK;
 }
 
+// The fast ADE's tile width of each member for this membership (the ADE's
+// results do not depend on it): the member's own while the batch's ADE blocks
+// fill the GPU's wave slots, halved (down to 16 columns) while they are fewer
+// than ADE_FILL.  That is the case when few members are left, whose waves are
+// then too few for the GPU and walk long column runs one after the other.
+constexpr int ADE_FILL = 4096;
+
+void ade_widths(fx_batch b)
+{
+    std::vector<int> w(b->maxm, 0);
+    auto blocks = [&](int s) {
+        const fx_bslot &B = b->slots_host[s];
+        return (int64_t)B.agx * ((B.P.ncols + w[s] - 1) / w[s]);
+    };
+    int64_t tot = 0;
+    for (int s = 0; s < b->maxm; s++)
+        if (b->member[s]) {
+            w[s] = fx_make_params(b->member[s]).ach;
+            tot += blocks(s);
+        }
+    for (bool narrower = true; tot < ADE_FILL && narrower;) {
+        narrower = false;
+        tot = 0;
+        for (int s = 0; s < b->maxm; s++) {
+            if (!b->member[s]) continue;
+            if (w[s] > 16) {
+                w[s] = std::max(16, w[s] / 2);
+                narrower = true;
+            }
+            tot += blocks(s);
+        }
+    }
+    for (int s = 0; s < b->maxm; s++)
+        if (b->member[s]) {
+            b->slots_host[s].P.ach = w[s];
+            b->nab[s] = (int)blocks(s);
+        }
+}
+
 // rebuild the slot table and the block -> member maps after a membership change
 int rebuild(fx_batch b)
 {
     std::vector<uint32_t> t, a;
     b->nslot = 0;
+    ade_widths(b);
     for (int s = 0; s < b->maxm; s++) {
         fx_domain d = b->member[s];
         if (!d) continue;

@@ -87,9 +87,10 @@ def _drive(batch: capi.Batch, t_end: dict, next_print: dict, print_step: float, 
 
 def run_ensemble(scens: list[gen.Scenario], inits: list[capi.InitialState], ade: bool = True,
                  steps_per_poll: int = 16, tiling: tuple[int, int] | None = None) -> dict:
-    """Admit every scenario into one batch and run all to their end time.
-    The timed region starts with every input resident on the device.  The
-    members' tile shape is `tiling`, or the autotuner's batch entry."""
+    """Admit every scenario into one batch and run all to their end time; a
+    member leaves the batch as it finishes.  The timed region starts with
+    every input resident on the device.  The members' tile shape is `tiling`,
+    or the autotuner's batch entry."""
     from . import autotune
     nchem = max(sc.nchem for sc in scens)
     batch = capi.Batch(nchem=nchem, ade=ade, max_members=len(scens))
@@ -101,21 +102,32 @@ def run_ensemble(scens: list[gen.Scenario], inits: list[capi.InitialState], ade:
         s = batch.admit(d, float(sc.t_end), float(sc.print_step))
         doms[s], t_end[s], next_print[s] = d, float(sc.t_end), float(sc.print_step)
     done: dict[int, int] = {}
+    solver_done: dict[int, int] = {}
     failed: dict[int, float] = {}
     cells_of = {s: d.nrows * d.ncols for s, d in doms.items()}
     total_cells = sum(cells_of.values())
     timeline = []                        # (wall s, members still running, their cells)
+
+    def finished(s, tim, steps, solver):
+        # a finished member leaves the batch at once, so that the steps of the
+        # others no longer launch its (idle) blocks
+        done[s], solver_done[s] = steps, solver
+        batch.evict(s)
+
+    def errored(s, tim):
+        failed[s] = tim
+        batch.evict(s)
+
     try:
         batch.poll()
         t0 = time.perf_counter()
         while len(done) + len(failed) < len(doms):
-            _drive(batch, t_end, next_print, float(scens[0].print_step),
-                   lambda s, tim, steps, solver: done.__setitem__(s, steps), steps_per_poll,
-                   lambda s, tim: failed.__setitem__(s, tim))
+            _drive(batch, t_end, next_print, float(scens[0].print_step), finished, steps_per_poll, errored)
             running = [s for s in doms if s not in done and s not in failed]
             timeline.append((time.perf_counter() - t0, len(running), sum(cells_of[s] for s in running)))
         el = time.perf_counter() - t0
-        n, cells, upd = batch.info()
+        n, cells = len(doms), total_cells
+        upd = sum(cells_of[s] * solver_done[s] for s in solver_done)
     finally:
         batch.close()
         for d in doms.values():
