@@ -615,7 +615,7 @@ def main() -> int:
         serving["reduced"] = args.serve_seconds < 300.0
         if serving["reduced"]:
             serving["note"] = (f"reduced: {args.serve_seconds:g}-s arrival window of the stated 300 s "
-                               "(the full stream is ~38 GPU-minutes at today's throughput)")
+                               "(the full stream is ~23 GPU-minutes of serving at today's throughput)")
 
     acc_cfg = None
     if args.accuracy_configs:

@@ -87,8 +87,8 @@ def _drive(batch: capi.Batch, t_end: dict, next_print: dict, print_step: float, 
 
 def run_ensemble(scens: list[gen.Scenario], inits: list[capi.InitialState], ade: bool = True,
                  steps_per_poll: int = 16, tiling: tuple[int, int] | None = None) -> dict:
-    """Admit every scenario into one batch and run all to their end time; a
-    member leaves the batch as it finishes.  The timed region starts with
+    """Admit every scenario into one batch and run each to its print at t_end
+    (its result); a member leaves the batch as it finishes.  The timed region starts with
     every input resident on the device.  The members' tile shape is `tiling`,
     or the autotuner's batch entry."""
     from . import autotune
@@ -320,3 +320,29 @@ def serve(requests: list[Request], budget_bytes: int = 200 * 10 ** 9, max_member
                 "cell_updates": int(upd), "cell_updates_per_s": upd / wall, "by_side": by_side,
                 "mean_steps": float(np.mean([r.steps for r in finished]))})
     return out
+
+
+def main(argv=None) -> int:
+    """``python -m fluxos_cpp_amd.serve --ensemble N`` runs the first N C3
+    scenarios in one batch; ``--stream RATE SECONDS`` serves a C4 Poisson
+    stream.  The result goes to stdout as one JSON line."""
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(prog="python -m fluxos_cpp_amd.serve")
+    g = ap.add_mutually_exclusive_group(required=True)
+    g.add_argument("--ensemble", type=int, metavar="N", help="the C3 ensemble of N scenarios")
+    g.add_argument("--stream", type=float, nargs=2, metavar=("RATE", "SECONDS"), help="a C4 arrival stream")
+    args = ap.parse_args(argv)
+    capi.load()
+    if args.ensemble:
+        scens = [gen.scenario_c3(i) for i in range(args.ensemble)]
+        inits = [capi.prepare(sc) for sc in scens]
+        out = run_ensemble(scens, inits)
+    else:
+        out = serve(c4_stream(rate=args.stream[0], seconds=args.stream[1]))
+    print(json.dumps(out, default=float), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -11,8 +11,9 @@ The FP64 parity path must match the oracle bit for bit at every print time:
 the oracle's trajectories are recorded in tests/golden/oracle_<case>.json
 (written by tests/golden/make_oracle_fixtures.py) as the clock and a SHA-256
 of every state grid.  Beside them: six C3 members in one batch, each
-bit-identical to its run alone, and the C5 generator on the FP32-mixed path
-with four solutes under two tile shapes, bit-identical."""
+bit-identical to its run alone, the ensemble driver's count of their cell
+updates, and the C5 generator on the FP32-mixed path with four solutes under
+two tile shapes, bit-identical."""
 from __future__ import annotations
 
 import json
@@ -108,6 +109,36 @@ def test_c3_members_in_one_batch_match_their_runs_alone():
     batch.close()
     for d in doms:
         d.close()
+
+
+def test_run_ensemble_counts_every_member():
+    """serve.run_ensemble, the ensemble leg's driver, on the same six members:
+    every member runs to its print at t_end (its result; the reference's loop
+    takes one more step after it, which nothing prints) and leaves the batch,
+    and the reported cell updates are each member's cells times the solver
+    steps of its run alone to that print."""
+    from fluxos_cpp_amd import serve
+    idx = (4, 22, 44, 54, 27, 58)
+    scs = [gen.scenario_c3(i) for i in idx]
+    inits = [capi.prepare(sc) for sc in scs]
+    expect = 0
+    for sc, init in zip(scs, inits):
+        d = capi.HipDomain.from_scenario(sc, precision=capi.FP32_MIXED, ade=True, init=init)
+        pn, solver = float(sc.print_step), 0
+        while True:
+            info = d.advance(float(sc.t_end), pn)
+            solver += info.solver_steps
+            if info.status & 2 or info.tim >= sc.t_end:
+                break
+            if info.status & 1:
+                pn += float(sc.print_step)
+        expect += sc.nrows * sc.ncols * solver
+        d.close()
+    r = serve.run_ensemble(scs, inits)
+    print(f"cell updates {r['cell_updates']} (alone: {expect}), {r['wall_s']:.2f} s", flush=True)
+    assert r["failed_slots"] == []
+    assert r["scenarios"] == len(scs)
+    assert r["cell_updates"] == expect
 
 
 def test_c5_generator_fp32_four_solutes_independent_of_tiling():
