@@ -389,13 +389,15 @@ __device__ inline void solver_wet32(const consts32 &K, const nb32 &n, const fron
 // their maximum, in FP32 (exact in any order, so the same for every tiling),
 // and the clock divides once (cfl_dt).  h is the depth or the signed
 // d = z - zb (the same above hdry > 0); the host holds hdry >= 1e-30, so hf is
-// a normal FP32 number.
+// a normal FP32 number.  The largest depth (hpall, main.cpp:312) is kept in
+// FP32 as well: the loop only tests it against zero (the solver gate), and
+// the FP32 maximum is exact in any order.
 __device__ inline void cfl_cell(const fx_params &P, float sqrt_g, double h, float qx, float qy, float &smax,
-                                double &hmx)
+                                float &hmx)
 {
     if (h > P.hdry) {
-        hmx = fmax(hmx, h);
         const float hf = (float)h;
+        hmx = fx_max_nn(hmx, hf);
         const float r = fx_rsq(hf);                          // 1/h = r^2, sqrt(g h) = sqrt(g) h r
         const float c0 = sqrt_g * (hf * r);
         const float u = fx_max_nn(1e-6f, fx_max_nn(fabsf(qx), fabsf(qy)) * (r * r));
@@ -583,7 +585,7 @@ __device__ inline void flux_col(const sweep_ctx &X, ring &R, const cmask &M, int
 
 // U(c): hydro loops 3 and 4 for this lane's own cell, and its wave speed
 template <int S0, int SS, int SN>
-__device__ inline void update_col(const sweep_ctx &X, ring &R, const cmask &M, int c, float &smax, double &hmx,
+__device__ inline void update_col(const sweep_ctx &X, ring &R, const cmask &M, int c, float &smax, float &hmx,
                                   double &wv, bool &bad)
 {
     if (!X.own) return;
@@ -688,7 +690,7 @@ __device__ inline void update_col(const sweep_ctx &X, ring &R, const cmask &M, i
 // arrays (column c0 + j uses j % 3); its two-column record is c & 1
 template <int SC>
 __device__ inline void column_step(const sweep_ctx &X, ring &R, cmask &M, int c, int c_last, pref &a,
-                                   float &smax, double &hmx, double &wv, double &ms, bool &bad)
+                                   float &smax, float &hmx, double &wv, double &ms, bool &bad)
 {
     constexpr int SN = (SC + 1) % 3, SS = (SC + 2) % 3;
     __syncthreads();
@@ -757,7 +759,8 @@ __device__ inline void sweep_tile(const fx_params &P, const s32p &S, fx_ctl *ctl
 
     pref a;
     float smax = 0.f;
-    double hmx = 0.0, wv = 0.0, ms = 0.0;
+    float hmx = 0.f;
+    double wv = 0.0, ms = 0.0;
     bool bad = false;
     // prologue: columns c0-2 (slot 1), c0-1 (slot 2), c0 (slot 0), fluxes of c0-1
     cmask M;
@@ -777,14 +780,14 @@ __device__ inline void sweep_tile(const fx_params &P, const s32p &S, fx_ctl *ctl
     if (c <= c_last) { column_step<0>(X, R, M, c, c_last, a, smax, hmx, wv, ms, bad); c++; }
     if (c <= c_last) { column_step<1>(X, R, M, c, c_last, a, smax, hmx, wv, ms, bad); c++; }
     smax = wmaxf_nn(smax);
-    hmx = wmax(hmx);
+    hmx = wmaxf_nn(hmx);
     wv = wsum(wv);
     ms = wsum(ms);
     if (__any(bad) && X.k == 0) atomicOr(&ctl->status, 4);
     if (X.k == 0) {
         const size_t bid = bx + (size_t)by * gx;
         S.partials[FX32_NPV * bid] = (double)smax;
-        S.partials[FX32_NPV * bid + 1] = hmx;
+        S.partials[FX32_NPV * bid + 1] = (double)hmx;
         S.partials[FX32_NPV * bid + 2] = wv;
         S.partials[FX32_NPV * bid + 3] = ms;
     }
@@ -819,8 +822,7 @@ __global__ __launch_bounds__(64) void k32_cfl(fx_params P, s32p S, fx_ctl *ctl)
     const int r0 = 1 + blockIdx.x * TH, c0 = 1 + blockIdx.y * P.ch;
     const int row = r0 - 2 + k;
     const float sqrt_g = fx_sqrt((float)P.g);
-    float smax = 0.f;
-    double hmx = 0.0;
+    float smax = 0.f, hmx = 0.f;
     if (k >= 2 && k <= NS - 2 && row <= P.nrows) {
         for (int c = c0; c <= min(c0 + P.ch - 1, P.ncols); c++) {
             const uint32_t p = (uint32_t)row + (uint32_t)c * (uint32_t)P.ld;
@@ -828,11 +830,11 @@ __global__ __launch_bounds__(64) void k32_cfl(fx_params P, s32p S, fx_ctl *ctl)
         }
     }
     smax = wmaxf_nn(smax);
-    hmx = wmax(hmx);
+    hmx = wmaxf_nn(hmx);
     if (k == 0) {
         const size_t bid = blockIdx.x + (size_t)blockIdx.y * gridDim.x;
         S.partials[FX32_NPV * bid] = (double)smax;
-        S.partials[FX32_NPV * bid + 1] = hmx;
+        S.partials[FX32_NPV * bid + 1] = (double)hmx;
     }
 }
 
