@@ -217,7 +217,10 @@ int fx_synchronize(fx_domain dom);
  * any time, calls that change a member's state, clock, tables, tiling or
  * WINTRA setting, the granular seams, fx_advance and fx_domain_destroy
  * return FX_ERR_STATE until it is evicted (the batch holds the member's
- * launch parameters and table addresses from admission on).                 */
+ * launch parameters and table addresses from admission on).  While the
+ * batch's fast-ADE blocks are too few to fill the GPU (few members left),
+ * it runs the members' ADE on narrower tiles than their own; results do not
+ * depend on the tiling.                                                      */
 typedef struct fx_batch_s *fx_batch;
 #define FX_EV_PRINT 1       /* member reached its print time (tim <= t_end)   */
 #define FX_EV_DONE  2       /* member's clock passed t_end (main.cpp:271)     */
