@@ -65,7 +65,7 @@ __device__ inline double wsum(double v)
 // kernels address this way is device memory, and a pointer read from memory
 // (a batch member's slot) would otherwise be a generic one, whose loads count
 // on the LDS counter as well, so that every LDS wait would also wait for them.
-// (the host pass of these inline device functions sees plain pointers)
+// The host pass of these inline device functions sees plain pointers.
 #if defined(__HIP_DEVICE_COMPILE__)
 #define FX_GLOBAL __attribute__((address_space(1)))
 #else
@@ -83,7 +83,7 @@ __device__ inline const FX_GLOBAL T &at_off(const T *base, uint32_t off)
 }
 
 // finiteness by the exponent bits, which no floating-point option of the
-// compiler can fold away (the FP32 kernels are built without NaN semantics)
+// compiler can fold away (the compiler emits them as one class test)
 __device__ inline bool fx_finite(double v)
 {
     return (((unsigned long long)__double_as_longlong(v) >> 52) & 0x7ffull) != 0x7ffull;
