@@ -2,12 +2,14 @@
 //
 // A batch holds up to max_members domains ("members").  One batched step
 // runs one iteration of the reference's loop (main.cpp:271-406) for every
-// member, each with its own CFL step, clock, forcing and print time, in four
+// member, each with its own CFL step, clock, forcing and print time, in three
 // launches that cover all members at once:
-//   k32b_clock   one block per member: fold its CFL partials, dt, forcing;
+//   k32b_clock   one block per member: the previous step's buffer flip and
+//                print/end halt, then fold its CFL partials, dt, forcing;
 //   k32b_sweep   every member's step-sweep blocks, packed in one grid;
 //   k32b_ade     every member's ADE blocks (if the batch carries solutes);
-//   k32b_end     buffer flip, print/end halt and a compact status per member.
+// and after a run's last step
+//   k32b_end     that step's flip and halt and a compact status per member.
 // The per-block member map (tmap/amap) is rebuilt when members join or
 // leave; a run of several steps is captured once as a hipGraph and replayed
 // (environment FLUXOS_BATCH_GRAPH=0 launches every step directly instead,
@@ -169,7 +171,7 @@ int launch_one(fx_batch b)
     if (b->ade && b->natiles > 0 &&
         (rc = fx32b_launch_ade(b->stream, b->slots_dev, b->amap, b->natiles, b->nchem)))
         return rc;
-    return fx32b_launch_end(b->stream, b->slots_dev, b->nslot, b->stat_dev);
+    return FX_OK;
 }
 
 int capture(fx_batch b)
@@ -295,6 +297,7 @@ int fx_batch_admit(fx_batch b, fx_domain d, double t_end, double print_next, int
     c->t_end = t_end;
     c->halt = c->tim > t_end ? 1 : 0;
     c->pending_halt = 0;
+    c->flip_pending = 0;
     c->steps = 0;
     c->solver_steps = 0;
     c->status &= ~4;
@@ -374,6 +377,9 @@ int fx_batch_run(fx_batch b, int32_t nsteps)
     }
     for (; left > 0; left--)
         if ((rc = launch_one(b))) return rc;
+    // each step's buffer flip and halt are done by the next step's clock; the
+    // last step's, and the status the host polls, here
+    if ((rc = fx32b_launch_end(b->stream, b->slots_dev, b->nslot, b->stat_dev))) return rc;
     b->steps_run += nsteps;
     return FX_OK;
 }

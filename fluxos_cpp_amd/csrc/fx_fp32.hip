@@ -870,7 +870,8 @@ __device__ inline void clock_fold(double *partials, int lo, int hi, double &a, d
         }
 }
 
-// single block (batched members)
+// single block (batched members); a member that steps leaves its buffer flip
+// and halt to the next clock (batch_step_close)
 __device__ inline void clock_body(const fx_params &P, double *partials, int nblk, fx_ctl *ctl)
 {
     if (ctl->halt) return;
@@ -881,6 +882,19 @@ __device__ inline void clock_body(const fx_params &P, double *partials, int nblk
     double a, b, c, m;
     clock_fold(partials, 0, nblk, a, b, c, m);
     clock_finish(P, ctl, a, b, c, m);
+    if (threadIdx.x == 0) ctl->flip_pending = 1;
+}
+
+// the end of a batch member's step (buffer flip, print / end / NaN halt: a
+// NaN/Inf member halts alone), done by the next step's clock or, after a
+// run's last step, by k32b_end
+__device__ inline void batch_step_close(fx_ctl *ctl)
+{
+    if (ctl->flip_pending) {
+        ctl->flip_pending = 0;
+        ctl->cur ^= 1;
+        ctl->halt = ctl->pending_halt | ((ctl->status >> 2) & 1);
+    }
 }
 
 // The end of a step (the buffer flip and the print / end / NaN halt), as the
@@ -1031,6 +1045,8 @@ __global__ void k32_flip_end(fx_ctl *ctl)
 __global__ __launch_bounds__(1024) void k32b_clock(const fx_bslot *slots)
 {
     const fx_bslot &B = slots[blockIdx.x];
+    if (threadIdx.x == 0) batch_step_close(B.ctl);    // the previous step's end
+    __syncthreads();
     clock_body(B.P, B.S.partials, B.npart, B.ctl);
 }
 
@@ -1058,17 +1074,14 @@ __global__ __launch_bounds__(256) void k32_wintra(fx_params P, a32p A, double *s
     soil[i] = m;
 }
 
-// batched end of step: the buffer flip and the print/end halt of every
-// member (k32_flip + k_step_end), and the compact status the host polls
+// batched end of a run: the last step's buffer flip and print/end halt of
+// every member (k32_flip + k_step_end), and the compact status the host polls
 __global__ void k32b_end(const fx_bslot *slots, int nslot, fx_bstat *stat)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nslot) return;
     fx_ctl *ctl = slots[i].ctl;
-    if (!ctl->halt) {
-        ctl->cur ^= 1;
-        ctl->halt = ctl->pending_halt | ((ctl->status >> 2) & 1);     // a NaN/Inf member halts alone
-    }
+    batch_step_close(ctl);                             // the last step's end
     fx_bstat st;
     st.tim = ctl->tim;
     st.dtfl = ctl->dtfl;
