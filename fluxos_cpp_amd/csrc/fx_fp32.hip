@@ -870,21 +870,6 @@ __device__ inline void clock_fold(double *partials, int lo, int hi, double &a, d
         }
 }
 
-// single block (batched members); a member that steps leaves its buffer flip
-// and halt to the next clock (batch_step_close)
-__device__ inline void clock_body(const fx_params &P, double *partials, int nblk, fx_ctl *ctl)
-{
-    if (ctl->halt) return;
-    if (ctl->tim > ctl->t_end) {                       // loop condition, main.cpp:271
-        if (threadIdx.x == 0) ctl->halt = 1;
-        return;
-    }
-    double a, b, c, m;
-    clock_fold(partials, 0, nblk, a, b, c, m);
-    clock_finish(P, ctl, a, b, c, m);
-    if (threadIdx.x == 0) ctl->flip_pending = 1;
-}
-
 // the end of a batch member's step (buffer flip, print / end / NaN halt: a
 // NaN/Inf member halts alone), done by the next step's clock or, after a
 // run's last step, by k32b_end
@@ -1042,12 +1027,43 @@ __global__ void k32_flip_end(fx_ctl *ctl)
 }
 
 // batched: one block per member
-__global__ __launch_bounds__(1024) void k32b_clock(const fx_bslot *slots)
+// One block per member, on a shared copy of its control block: the forcing
+// tables' times are loaded with it (when they fit one entry per thread), and
+// the copy is written back only if it changed (free slots share one halted
+// control block)
+__global__ __launch_bounds__(1024) void k32b_clock(const fx_bslot *__restrict__ slots)
 {
+    __shared__ fx_ctl C;
+    __shared__ int go_s, dirty_s;
     const fx_bslot &B = slots[blockIdx.x];
-    if (threadIdx.x == 0) batch_step_close(B.ctl);    // the previous step's end
+    const fx_params &P = B.P;
+    const bool held = (!P.meteo || P.meteo_nrows <= (int)blockDim.x) &&
+                      (!P.inflow || P.inflow_nrows <= (int)blockDim.x);
+    float mt = 0.f, it = 0.f;
+    if (held && P.meteo && (int)threadIdx.x < P.meteo_nrows) mt = P.meteo[(size_t)threadIdx.x * P.meteo_ncols];
+    if (held && P.inflow && (int)threadIdx.x < P.inflow_nrows) it = P.inflow[(size_t)threadIdx.x * P.inflow_ncols];
+    ctl_load(C, B.ctl);
     __syncthreads();
-    clock_body(B.P, B.S.partials, B.npart, B.ctl);
+    if (threadIdx.x == 0) {
+        const int was_pending = C.flip_pending, was_halt = C.halt;
+        batch_step_close(&C);                          // the previous step's end
+        int go = !C.halt;
+        if (go && C.tim > C.t_end) {                   // loop condition, main.cpp:271
+            C.halt = 1;
+            go = 0;
+        }
+        go_s = go;
+        dirty_s = go || was_pending || C.halt != was_halt;
+    }
+    __syncthreads();
+    if (go_s) {
+        double a, b, c, m;
+        clock_fold(B.S.partials, 0, B.npart, a, b, c, m);
+        clock_finish_held(P, &C, a, b, c, m, held, mt, it);
+        if (threadIdx.x == 0) C.flip_pending = 1;      // this step's end: the next clock's
+    }
+    __syncthreads();
+    if (dirty_s) ctl_store(B.ctl, C);
 }
 
 // WINTRAsolver_calc.cpp:26-57 on the step's new state (before the flip):
